@@ -1,0 +1,3 @@
+from .projection import Radon, Fanbeam
+from .base import IDENTITY, ZERO
+from .hip import HipKernelSpec

@@ -1,0 +1,182 @@
+/*
+ * gratopy_hip.h -- C ABI of the MI355X (gfx950) projection library.
+ *
+ * Every compute entry point is asynchronous on the HIP stream it is given
+ * (pass NULL for the null stream), takes raw device pointers and plain sizes,
+ * allocates nothing, and returns a status code (GT_OK on success).  The text
+ * of the last failure of the calling thread is available from
+ * gt_last_error().  Arrays are dense and either Fortran ordered (GT_ORDER_F:
+ * x (or s) fastest, z slowest) or C ordered (GT_ORDER_C: z fastest), exactly
+ * as in the reference's pos_img_f / pos_img_c / pos_sino_f / pos_sino_c
+ * macros (gratopy/radon.cl:28-31).
+ *
+ * Geometry tables (ofs, sdpd, geo, angle_weights) are the reference's device
+ * buffers, byte for byte: the layouts written by radon_struct
+ * (gratopy/gratopy.py:553-569) and fanbeam_struct (gratopy/gratopy.py:950-988),
+ * converted to the element type of the call.
+ *
+ * Each projection entry point needs a device workspace of at least
+ * gt_projection_workspace_bytes(...) bytes (a transposed image copy for the
+ * forward direction; a zero-padded, pre-weighted sinogram copy and a per-angle
+ * table for the backward direction).
+ */
+#ifndef GRATOPY_HIP_H
+#define GRATOPY_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *gt_stream_t; /* a hipStream_t */
+
+enum { GT_FLOAT32 = 0, GT_FLOAT64 = 1 };
+enum { GT_ORDER_F = 0, GT_ORDER_C = 1 };
+enum { GT_GEOMETRY_RADON = 0, GT_GEOMETRY_FAN = 1 };
+enum { GT_OK = 0, GT_EINVAL = 1, GT_ENOSPC = 2, GT_EHIP = 3 };
+
+/* Library version (major*10000 + minor*100 + patch). */
+int gt_version(void);
+
+/* Copies the calling thread's last error message into buf (NUL terminated,
+ * truncated to n bytes); returns the full message length. */
+size_t gt_last_error(char *buf, size_t n);
+
+/* Workspace bytes needed by gt_{radon,fan}_{fwd,back} for these sizes.
+ * adjoint = 0 for the forward projection, 1 for the backprojection;
+ * in_order is the order of the input array of that call. */
+size_t gt_projection_workspace_bytes(int geometry, int adjoint, int dtype, int Nx, int Ny,
+                                     int Ns, int Na, int Nz, int in_order);
+
+/* Parallel-beam forward projection sino = R img.
+ * Replaces kernel radon_<t>_<sino_order><img_order> (gratopy/radon.cl:66-161),
+ * launched by gratopy.radon (gratopy/gratopy.py:258-316). */
+int gt_radon_fwd(gt_stream_t stream, int dtype, void *sino, const void *img, const void *ofs,
+                 const void *geo, int Nx, int Ny, int Ns, int Na, int Nz, int sino_order,
+                 int img_order, void *workspace, size_t workspace_bytes);
+
+/* Parallel-beam backprojection img = R^# sino.
+ * Replaces kernel radon_ad_<t>_<img_order><sino_order> (gratopy/radon.cl:183-241),
+ * launched by gratopy.radon_ad (gratopy/gratopy.py:319-377). */
+int gt_radon_back(gt_stream_t stream, int dtype, void *img, const void *sino, const void *ofs,
+                  const void *geo, int Nx, int Ny, int Ns, int Na, int Nz, int img_order,
+                  int sino_order, void *workspace, size_t workspace_bytes);
+
+/* Fanbeam forward projection.
+ * Replaces kernel fanbeam_<t>_<sino_order><img_order> (gratopy/fanbeam.cl:72-256),
+ * launched by gratopy.fanbeam (gratopy/gratopy.py:581-640). */
+int gt_fan_fwd(gt_stream_t stream, int dtype, void *sino, const void *img, const void *ofs,
+               const void *sdpd, const void *geo, int Nx, int Ny, int Ns, int Na, int Nz,
+               int sino_order, int img_order, void *workspace, size_t workspace_bytes);
+
+/* Fanbeam backprojection.
+ * Replaces kernel fanbeam_ad_<t>_<img_order><sino_order> (gratopy/fanbeam.cl:290-391),
+ * launched by gratopy.fanbeam_ad (gratopy/gratopy.py:643-703). */
+int gt_fan_back(gt_stream_t stream, int dtype, void *img, const void *sino, const void *ofs,
+                const void *sdpd, const void *geo, int Nx, int Ny, int Ns, int Na, int Nz,
+                int img_order, int sino_order, void *workspace, size_t workspace_bytes);
+
+/* Sinogram of a unit impulse at pixel (x, y): one column of the system matrix.
+ * sdpd is ignored (may be NULL) for the parallel geometry.
+ * Replaces single_line_radon_* (gratopy/radon.cl:263-341) and
+ * single_line_fanbeam_* (gratopy/fanbeam.cl:422-568), used by
+ * ProjectionSettings.create_sparse_matrix (gratopy/gratopy.py:2014-2155). */
+int gt_single_line(gt_stream_t stream, int geometry, int dtype, void *sino, int x, int y,
+                   const void *ofs, const void *sdpd, const void *geo, int Nx, int Ny, int Ns,
+                   int Na, int sino_order);
+
+/* out[s,a,z] = in[s,a,z] * w[a]  (divide != 0: / w[a]); out may alias in.
+ * Replaces multiply_* / divide_* (gratopy/utilities.cl:38-67) as launched by
+ * gratopy.weight_sinogram (gratopy/gratopy.py:2188-2266). */
+int gt_weight_sino(gt_stream_t stream, int dtype, void *out, const void *in,
+                   const void *angle_weights, int Ns, int Na, int Nz, int order, int divide);
+
+/* rhs += a*x over n elements.  Replaces equ_mul_add_* (gratopy/utilities.cl:69-73),
+ * gratopy.equ_mul_add (gratopy/gratopy.py:2269-2295).  a is rounded to the
+ * element type before use, as the reference passes it as a `real` argument. */
+int gt_axpy(gt_stream_t stream, int dtype, void *rhs, double a, const void *x, size_t n);
+
+/* rhs = a*x + y.  Replaces mul_add_add_* (gratopy/utilities.cl:75-80),
+ * gratopy.mul_add_add (gratopy/gratopy.py:2298-2326). */
+int gt_axpby(gt_stream_t stream, int dtype, void *rhs, double a, const void *x, const void *y,
+             size_t n);
+
+/* As gt_axpy / gt_axpby, with the scalar read on the device: a = mult * (*a_dev)
+ * (a_dev points to one double).  No reference counterpart: lets the
+ * conjugate-gradient loop (gratopy/gratopy.py:2519-2548) keep alpha and beta on
+ * the device so that its iterations can be captured in a HIP graph. */
+int gt_axpy_dev(gt_stream_t stream, int dtype, void *rhs, const void *a_dev, double mult,
+                const void *x, size_t n);
+int gt_axpby_dev(gt_stream_t stream, int dtype, void *rhs, const void *a_dev, double mult,
+                 const void *x, const void *y, size_t n);
+
+/* x /= (*s_dev) (divide != 0) or x *= (*s_dev), s_dev a device double.
+ * Replaces the host-synchronised `img /= normsqr` of gratopy.normest
+ * (gratopy/gratopy.py:2370-2376). */
+int gt_scale_dev(gt_stream_t stream, int dtype, void *x, const void *s_dev, int divide, size_t n);
+
+/* x *= a over n elements.  Replaces the pyopencl `output *= scalar` of
+ * _OpenCLOperator.apply_to (gratopy/operator/opencl.py:436-437). */
+int gt_scale(gt_stream_t stream, int dtype, void *x, double a, size_t n);
+
+/* x[:] = value (a fill kernel; never a memset node). */
+int gt_fill(gt_stream_t stream, int dtype, void *x, double value, size_t n);
+
+/* out = a - b elementwise (out may alias a).  Replaces the pyopencl
+ * elementwise `sinonew -= sino` of gratopy.landweber (gratopy/gratopy.py:2436). */
+int gt_sub(gt_stream_t stream, int dtype, void *out, const void *a, const void *b, size_t n);
+
+/* Workspace bytes for gt_dot / gt_sum over n elements. */
+size_t gt_reduce_workspace_bytes(size_t n);
+
+/* *out_dev (a device double) = sum_i x[i]*y[i], accumulated in double with a
+ * fixed reduction tree (deterministic).  Replaces pyopencl.array.vdot / dot as
+ * used by gratopy.conjugate_gradients (gratopy/gratopy.py:2527-2543). */
+int gt_dot(gt_stream_t stream, int dtype, void *out_dev, const void *x, const void *y, size_t n,
+           void *workspace, size_t workspace_bytes);
+
+/* *out_dev = sum_i x[i].  Replaces pyopencl.array.sum in gratopy.normest
+ * (gratopy/gratopy.py:2372). */
+int gt_sum(gt_stream_t stream, int dtype, void *out_dev, const void *x, size_t n, void *workspace,
+           size_t workspace_bytes);
+
+/* Conjugate-gradient scalar update on device doubles d[0..7]:
+ * mode 0: d[4] = scale * d[0] / d[1]   (alpha = dx^2/ds <sold,sold>/<Wq,q>)
+ * mode 1: d[5] = d[2] / d[0]           (beta = <snew,snew>/<sold,sold>), then d[0] = d[2]
+ * The call performs mode 0 when scale > 0 and mode 1 when scale <= 0.
+ * Mirrors the host arithmetic of gratopy.conjugate_gradients (gratopy/gratopy.py:2523-2538). */
+int gt_cg_scalars(gt_stream_t stream, void *d, double scale);
+
+/* TV primal-dual pointwise updates, one per reference kernel
+ * (gratopy/total_variation.cl).  v is the dual variable, 4 elements per pixel
+ * (a padded 3-vector), indexed by the pixel's position in the image order.
+ * sigma, tau, ... are float as in the reference (gratopy/gratopy.py:2647-2685). */
+int gt_tv_update_v(gt_stream_t stream, int dtype, void *v, const void *u, float sigma,
+                   float z_distance, int Nx, int Ny, int Nz, int order); /* total_variation.cl:54-93 */
+int gt_tv_update_normv(gt_stream_t stream, int dtype, void *v, void *normv, int Nx, int Ny,
+                       int Nz, int order); /* total_variation.cl:150-167 */
+int gt_tv_update_lambda(gt_stream_t stream, int dtype, void *lambda, const void *ku,
+                        const void *f, float sigma, float sigmap1inv, int Ns, int Na, int Nz,
+                        int order); /* total_variation.cl:95-106 */
+int gt_tv_update_u(gt_stream_t stream, int dtype, void *u, const void *u_old, const void *v,
+                   const void *kstarlambda, float tau, float norming, float z_distance, int Nx,
+                   int Ny, int Nz, int order); /* total_variation.cl:108-148 */
+int gt_tv_update_extra(gt_stream_t stream, int dtype, void *u_, void *u,
+                       size_t n); /* u = 2 u_ - u; the ElementwiseKernel of gratopy.py:2708-2717 */
+
+/* Fused TV updates (same arithmetic, fewer passes over HBM):
+ * gt_tv_dual   = update_v followed by update_NormV_unchor;
+ * gt_tv_primal = update_u (into u_new) followed by the extragradient step
+ *                u_old <- 2 u_new - u_old.  */
+int gt_tv_dual(gt_stream_t stream, int dtype, void *v, const void *u, float sigma,
+               float z_distance, int Nx, int Ny, int Nz, int order);
+int gt_tv_primal(gt_stream_t stream, int dtype, void *u_new, void *u_old, const void *v,
+                 const void *kstarlambda, float tau, float norming, float z_distance, int Nx,
+                 int Ny, int Nz, int order);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRATOPY_HIP_H */
