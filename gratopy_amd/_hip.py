@@ -40,6 +40,8 @@ _f = ctypes.c_float
 # name -> (restype, argtypes); kept in sync with include/gratopy_hip.h
 PROTOTYPES: dict[str, tuple] = {
     "gt_version": (_i, []),
+    "gt_tune_set": (_i, [_i, _i]),
+    "gt_tune_get": (_i, [_i]),
     "gt_last_error": (_sz, [ctypes.c_char_p, _sz]),
     "gt_projection_workspace_bytes": (_sz, [_i, _i, _i, _i, _i, _i, _i, _i, _i]),
     "gt_radon_fwd": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _vp, _sz]),

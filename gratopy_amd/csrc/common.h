@@ -21,6 +21,13 @@ int check_launch(const char *what);
 
 inline size_t align_up(size_t n, size_t a = 256) { return (n + a - 1) / a * a; }
 
+// ---- launch-shape knobs set by the autotuner (gt_tune_set) -----------------
+struct Tuning {
+  int fwd_group = 4;  // detector bins per lane in the forward kernels (1, 2, 4, 8)
+  int back_rows = 1;  // pixels per lane (along y) in the backprojection (1, 2, 4)
+};
+Tuning &tuning();
+
 // ---- array orders (gratopy/radon.cl:28-31) --------------------------------
 struct Strides3 {
   int64_t s0, s1, s2;  // stride of index 0 (x or s), index 1 (y or a), index 2 (z)

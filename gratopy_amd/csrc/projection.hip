@@ -73,8 +73,9 @@ __device__ inline void clip_rows(double lo, double hi, int Nyy, int &ylo, int &y
     yhi = Nyy - 1;
     return;
   }
-  lo = fmax(lo - 1.0, -1.0);
-  hi = fmin(hi + 1.0, (double)Nyy);
+  // clamp in floating point first: infinities must not reach the int conversion
+  lo = fmin(fmax(lo - 1.0, -1.0), (double)Nyy);
+  hi = fmax(fmin(hi + 1.0, (double)Nyy), -1.0);
   ylo = (int)floor(lo);
   yhi = (int)ceil(hi);
   ylo = ylo < 0 ? 0 : ylo;
@@ -259,8 +260,10 @@ __global__ __launch_bounds__(256) void fan_fwd_kernel(real *__restrict__ sino, S
     const real u = (real)y - midy;
     const real xhigh = qx + dpx * (u - qy);
     const real xlow = qx + dmx * (u - qy);
-    const int xh = (int)floor(fmin(xcap, xhigh) + midx);
-    const int xl = (int)ceil(fmax(-midx, xlow) + midx);
+    // bounds of the strip in this row; the outer fmax/fmin only keep
+    // infinities (rays parallel to the rows) away from the int conversion
+    const int xh = (int)floor(fmax(fmin(xcap, xhigh) + midx, (real)-1));
+    const int xl = (int)ceil(fmin(fmax(-midx, xlow) + midx, (real)Nxx));
     const real xr = (real)xl - midx;
     real t = ddx * xr + ddy * u + RE;
     real ss = dx * xr + dy * u;
@@ -275,6 +278,213 @@ __global__ __launch_bounds__(256) void fan_fwd_kernel(real *__restrict__ sino, S
     }
   }
   sino[s * so.s0 + a * so.s1 + z * so.s2] = acc * sdpd[s] / delta_xi * delta_x / R;
+}
+
+// ---------------------------------------------------------------------------
+// Forward projections, bin-group form.  A lane owns K adjacent detector bins
+// [s0, s0+K) of one angle and walks, row by row, the union strip between the
+// rays through detector positions s0-1 and s0+K.  Each pixel of the strip is
+// projected once; its two hat weights (1-f, f) go to the bins floor(p) and
+// floor(p)+1 it touches, accumulated in lane-private LDS slots (slot k+2 holds
+// bin s0+k; slots outside [2, K+2) collect the neighbours' share and are
+// dropped).  Per bin, the summation order is the reference's (rows ascending,
+// then along the row), so results agree with the per-ray loops to rounding;
+// the per-row bound computation is shared by K rays and each pixel is
+// projected once instead of twice.
+// ---------------------------------------------------------------------------
+template <int K>
+struct Slots {
+  static constexpr int kCount = K + 4;
+};
+
+template <typename real, int K>
+__global__ __launch_bounds__(256) void fan_fwd_group_kernel(
+    real *__restrict__ sino, Strides3 so, const real *__restrict__ imgX,
+    const real *__restrict__ imgY, const real *__restrict__ ofs, const real *__restrict__ sdpd,
+    const real *__restrict__ geo, int Nx, int Ny, int Ns, int Na) {
+  using A = Arith<real>;
+  __shared__ real slots[Slots<K>::kCount * 256];
+  const int tid = threadIdx.y * 64 + threadIdx.x;
+  const int s0 = (blockIdx.x * 64 + threadIdx.x) * K;
+  const int a = blockIdx.y * 4 + threadIdx.y;
+  const int z = blockIdx.z;
+  if (s0 >= Ns || a >= Na) return;
+  real *acc = slots + tid;
+#pragma unroll
+  for (int k = 0; k < Slots<K>::kCount; ++k) acc[k * 256] = 0;
+
+  const real R = geo[0];
+  real RE = geo[1];
+  const real delta_xi = geo[2];
+  real midx = geo[3], midy = geo[4];
+  const real mid_det = geo[5];
+  const real delta_x = geo[10];
+  const real *o = ofs + 8 * (int64_t)a;
+  real dx = o[0], dy = o[1], qx = o[2], qy = o[3];
+  const real d0x = o[4], d0y = o[5];
+  // rays through detector positions s0 + K (upper) and s0 - 1 (lower)
+  const real sp = -mid_det + (real)(s0 + K);
+  const real sm = -mid_det + (real)(s0 - 1);
+  real dpx = d0x + dx * sp - qx, dpy = d0y + dy * sp - qy;
+  real dmx = d0x + dx * sm - qx, dmy = d0y + dy * sm - qy;
+  real ddx = d0x - qx, ddy = d0y - qy;
+  const real RR = R * R;
+  ddx /= RR;
+  ddy /= RR;
+  const real dxi2 = delta_xi * delta_xi;
+  dx /= dxi2;
+  dy /= dxi2;
+  RE *= (real)1 / R;
+  int Nxx = Nx, Nyy = Ny;
+  const real *base;
+  int64_t row_stride;
+  const int64_t slice = (int64_t)z * Nx * Ny;
+  if (fabs(dpx) < fabs(dpy) && fabs(dmx) < fabs(dmy)) {
+    base = imgX + slice;
+    row_stride = Nx;
+  } else {
+    real t;
+    t = ddx; ddx = ddy; ddy = t;
+    t = dx; dx = dy; dy = t;
+    t = qx; qx = qy; qy = t;
+    t = dpx; dpx = dpy; dpy = t;
+    t = dmx; dmx = dmy; dmy = t;
+    t = midx; midx = midy; midy = t;
+    Nyy = Nx;
+    Nxx = Ny;
+    base = imgY + slice;
+    row_stride = Ny;
+  }
+  dpx /= dpy;
+  dmx /= dmy;
+  if (qy * (dmx - dpx) < (real)0) {
+    const real t = dpx;
+    dpx = dmx;
+    dmx = t;
+  }
+  int ylo, yhi;
+  fan_rows((double)qx, (double)qy, (double)dpx, (double)dmx, (double)midx, (double)midy, Nxx,
+           Nyy, ylo, yhi);
+  const real xcap = (real)(Nxx - 1) - midx;
+  const real c0 = mid_det - (real)s0;  // detector position relative to s0
+  const real lo = (real)-1.5, hi = (real)K + (real)0.5;
+  for (int y = ylo; y <= yhi; ++y) {
+    const real u = (real)y - midy;
+    const real xhigh = qx + dpx * (u - qy);
+    const real xlow = qx + dmx * (u - qy);
+    const int xh = (int)floor(fmax(fmin(xcap, xhigh) + midx, (real)-1));
+    const int xl = (int)ceil(fmin(fmax(-midx, xlow) + midx, (real)Nxx));
+    const real xr = (real)xl - midx;
+    real t = ddx * xr + ddy * u + RE;
+    real ss = dx * xr + dy * u;
+    const real *row = base + (int64_t)y * row_stride;
+    for (int x = xl; x <= xh; ++x) {
+      const real r = A::rcp(t);
+      const real p = A::clamp(ss * r + c0, lo, hi);
+      const real fl = floor(p);
+      const int j = (int)fl;
+      const real f = p - fl;
+      const real v = row[x] * r;
+      real *slot = acc + (j + 2) * 256;
+      slot[0] += v - f * v;
+      slot[256] += f * v;
+      t += ddx;
+      ss += dx;
+    }
+  }
+  const real scale_tail = delta_x / R;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int s = s0 + k;
+    if (s < Ns) sino[s * so.s0 + a * so.s1 + z * so.s2] = acc[(k + 2) * 256] * sdpd[s] / delta_xi * scale_tail;
+  }
+}
+
+template <typename real, int K>
+__global__ __launch_bounds__(256) void radon_fwd_group_kernel(
+    real *__restrict__ sino, Strides3 so, const real *__restrict__ imgX,
+    const real *__restrict__ imgY, const real *__restrict__ ofs, const real *__restrict__ geo,
+    int Nx, int Ny, int Ns, int Na) {
+  using A = Arith<real>;
+  __shared__ real slots[Slots<K>::kCount * 256];
+  const int tid = threadIdx.y * 64 + threadIdx.x;
+  const int s0 = (blockIdx.x * 64 + threadIdx.x) * K;
+  const int a = blockIdx.y * 4 + threadIdx.y;
+  const int z = blockIdx.z;
+  if (s0 >= Ns || a >= Na) return;
+  real *acc = slots + tid;
+#pragma unroll
+  for (int k = 0; k < Slots<K>::kCount; ++k) acc[k * 256] = 0;
+  const float delta_x = (float)geo[0];
+  const float delta_xi = (float)geo[1];
+  const real *o = ofs + 8 * (int64_t)a;
+  real X = o[0], Y = o[1];
+  const real off = o[2], inv = o[3];
+  int Nxx = Nx, Nyy = Ny;
+  const real *base;
+  int64_t row_stride;
+  const int64_t slice = (int64_t)z * Nx * Ny;
+  if (o[5] != (real)0) {
+    const real t = X;
+    X = Y;
+    Y = t;
+    Nxx = Ny;
+    Nyy = Nx;
+    base = imgY + slice;
+    row_stride = Ny;
+  } else {
+    base = imgX + slice;
+    row_stride = Nx;
+  }
+  // union strip of the bins [s0, s0+K): centre bin s0 + (K-1)/2, half width (K+1)/2
+  int ylo, yhi;
+  {
+    const double half = 0.5 * (K + 1);
+    const double Xd = X, Yd = Y;
+    const double h = half / fabs(Xd);
+    const double g0 = -((double)off - ((double)s0 + 0.5 * (K - 1))) / Xd, m = -Yd / Xd;
+    double rlo = -INFINITY, rhi = INFINITY;
+    restrict_ge(m, -1.0 - h - g0, rlo, rhi);
+    restrict_ge(-m, g0 - ((double)Nxx + h), rlo, rhi);
+    if (rlo > rhi) {
+      ylo = 0;
+      yhi = -1;
+    } else {
+      clip_rows(rlo, rhi, Nyy, ylo, yhi);
+    }
+  }
+  const real lo = (real)-1.5, hi = (real)K + (real)0.5;
+  const real sl = (real)(s0 - 1), sh = (real)(s0 + K);
+  for (int y = ylo; y <= yhi; ++y) {
+    const real d = (real)y * Y + off;  // detector position of (0, y)
+    int xl = (int)((sl - d) * inv);
+    int xh = (int)((sh - d) * inv);
+    if (inv < (real)0) {
+      const int t = xl;
+      xl = xh;
+      xh = t;
+    }
+    xl = xl < 0 ? 0 : xl;
+    xh = xh > Nxx - 1 ? Nxx - 1 : xh;
+    const real c = d - (real)s0;
+    const real *row = base + (int64_t)y * row_stride;
+    for (int x = xl; x <= xh; ++x) {
+      const real p = A::clamp((real)x * X + c, lo, hi);
+      const real fl = floor(p);
+      const int j = (int)fl;
+      const real f = p - fl;
+      const real v = row[x];
+      real *slot = acc + (j + 2) * 256;
+      slot[0] += v - f * v;
+      slot[256] += f * v;
+    }
+  }
+  const real scale = (real)delta_x * (real)delta_x / (real)delta_xi;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int s = s0 + k;
+    if (s < Ns) sino[s * so.s0 + a * so.s1 + z * so.s2] = acc[(k + 2) * 256] * scale;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -474,8 +684,8 @@ __global__ void single_line_fan_kernel(real *__restrict__ sino, Strides3 so, int
   const real u = (real)y - midy;
   const real xhigh = qx + dpx * (u - qy);
   const real xlow = qx + dmx * (u - qy);
-  const int xh = (int)floor(fmin((real)(Nxx - 1) - midx, xhigh) + midx);
-  const int xl = (int)ceil(fmax(-midx, xlow) + midx);
+  const int xh = (int)floor(fmax(fmin((real)(Nxx - 1) - midx, xhigh) + midx, (real)-1));
+  const int xl = (int)ceil(fmin(fmax(-midx, xlow) + midx, (real)Nxx));
   real acc = 0;
   if (xl <= x && x <= xh) {
     const real t = ddx * ((real)x - midx) + ddy * u + RE;
@@ -542,13 +752,36 @@ static int fwd_impl(hipStream_t st, real *sino, const real *img, const real *ofs
     imgY = cp;
   }
   const Strides3 so = order_strides(sino_order, L.Ns, L.Na, L.Nz);
-  dim3 grid((L.Ns + 63) / 64, (L.Na + 3) / 4, L.Nz), block(64, 4);
-  if (FAN)
-    hipLaunchKernelGGL(fan_fwd_kernel<real>, grid, block, 0, st, sino, so, imgX, imgY, ofs, sdpd,
-                       geo, L.Nx, L.Ny, L.Ns, L.Na);
-  else
-    hipLaunchKernelGGL(radon_fwd_kernel<real>, grid, block, 0, st, sino, so, imgX, imgY, ofs, geo,
-                       L.Nx, L.Ny, L.Ns, L.Na);
+  const int K = tuning().fwd_group;
+  dim3 block(64, 4);
+  dim3 grid((L.Ns + 64 * K - 1) / (64 * K), (L.Na + 3) / 4, L.Nz);
+  if (FAN) {
+    if (K == 8)
+      hipLaunchKernelGGL((fan_fwd_group_kernel<real, 8>), grid, block, 0, st, sino, so, imgX, imgY,
+                         ofs, sdpd, geo, L.Nx, L.Ny, L.Ns, L.Na);
+    else if (K == 4)
+      hipLaunchKernelGGL((fan_fwd_group_kernel<real, 4>), grid, block, 0, st, sino, so, imgX, imgY,
+                         ofs, sdpd, geo, L.Nx, L.Ny, L.Ns, L.Na);
+    else if (K == 2)
+      hipLaunchKernelGGL((fan_fwd_group_kernel<real, 2>), grid, block, 0, st, sino, so, imgX, imgY,
+                         ofs, sdpd, geo, L.Nx, L.Ny, L.Ns, L.Na);
+    else
+      hipLaunchKernelGGL(fan_fwd_kernel<real>, grid, block, 0, st, sino, so, imgX, imgY, ofs, sdpd,
+                         geo, L.Nx, L.Ny, L.Ns, L.Na);
+  } else {
+    if (K == 8)
+      hipLaunchKernelGGL((radon_fwd_group_kernel<real, 8>), grid, block, 0, st, sino, so, imgX,
+                         imgY, ofs, geo, L.Nx, L.Ny, L.Ns, L.Na);
+    else if (K == 4)
+      hipLaunchKernelGGL((radon_fwd_group_kernel<real, 4>), grid, block, 0, st, sino, so, imgX,
+                         imgY, ofs, geo, L.Nx, L.Ny, L.Ns, L.Na);
+    else if (K == 2)
+      hipLaunchKernelGGL((radon_fwd_group_kernel<real, 2>), grid, block, 0, st, sino, so, imgX,
+                         imgY, ofs, geo, L.Nx, L.Ny, L.Ns, L.Na);
+    else
+      hipLaunchKernelGGL(radon_fwd_kernel<real>, grid, block, 0, st, sino, so, imgX, imgY, ofs,
+                         geo, L.Nx, L.Ny, L.Ns, L.Na);
+  }
   return check_launch(FAN ? "fan forward projection" : "radon forward projection");
 }
 

@@ -22,9 +22,37 @@ int check_launch(const char *what) {
   return GT_OK;
 }
 
+Tuning &tuning() {
+  static Tuning t;
+  return t;
+}
+
 }  // namespace gt
 
 extern "C" int gt_version(void) { return 10000; }
+
+extern "C" int gt_tune_set(int key, int value) {
+  gt::Tuning &t = gt::tuning();
+  switch (key) {
+    case GT_TUNE_FWD_GROUP:
+      if (value != 1 && value != 2 && value != 4 && value != 8)
+        return gt::fail(GT_EINVAL, "forward group size must be 1, 2, 4 or 8");
+      t.fwd_group = value;
+      return GT_OK;
+    case GT_TUNE_BACK_ROWS:
+      if (value != 1 && value != 2 && value != 4)
+        return gt::fail(GT_EINVAL, "backprojection rows per lane must be 1, 2 or 4");
+      t.back_rows = value;
+      return GT_OK;
+    default:
+      return gt::fail(GT_EINVAL, "unknown tuning key");
+  }
+}
+
+extern "C" int gt_tune_get(int key) {
+  const gt::Tuning &t = gt::tuning();
+  return key == GT_TUNE_FWD_GROUP ? t.fwd_group : key == GT_TUNE_BACK_ROWS ? t.back_rows : -1;
+}
 
 extern "C" size_t gt_last_error(char *buf, size_t n) {
   const std::string &msg = gt::g_last_error;

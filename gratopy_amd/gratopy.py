@@ -493,11 +493,11 @@ class ProjectionSettings:
                 image_width=image_width, midpoint_shift=self.midpoint_shift,
                 reverse_detector=self.reverse_detector,
             )
-            self.ofs_buf = self.struct["ofs_dict"]
-            self.sdpd_buf = self.struct["sdpd_dict"]
+            self.ofs_buf = dict(self.struct["ofs_dict"])
+            self.sdpd_buf = dict(self.struct["sdpd_dict"])
             self.image_width = self.struct["image_width"]
-            self.geometry_information = self.struct["geo_dict"]
-            self.angle_weights_buf = self.struct["angle_diff_dict"]
+            self.geometry_information = dict(self.struct["geo_dict"])
+            self.angle_weights_buf = dict(self.struct["angle_diff_dict"])
             self.angle_weights = self.angle_weights_buf[np.dtype("float")].copy()
             self.delta_x = float(self.image_width) / max(self.img_shape)
             self.delta_s = float(detector_width) / self.n_detectors
@@ -511,12 +511,12 @@ class ProjectionSettings:
                 detector_width=self.detector_width, image_width=self.image_width,
                 midpoint_shift=self.midpoint_shift, detector_shift=self.detector_shift,
             )
-            self.ofs_buf = self.struct["ofs_dict"]
+            self.ofs_buf = dict(self.struct["ofs_dict"])
             self.delta_x = self.image_width / max(self.img_shape)
             self.delta_s = self.detector_width / self.n_detectors
             self.delta_ratio = self.delta_s / self.delta_x
-            self.geometry_information = self.struct["geo_dict"]
-            self.angle_weights_buf = self.struct["angle_diff_dict"]
+            self.geometry_information = dict(self.struct["geo_dict"])
+            self.angle_weights_buf = dict(self.struct["angle_diff_dict"])
             self.angle_weights = self.angle_weights_buf[np.dtype("float")].copy()
 
     @property

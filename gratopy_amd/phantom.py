@@ -37,16 +37,33 @@ def ct_modified_shepp_logan_params_2d():
 
 
 def rasterize_ellipses(M: int, N: int, E: np.ndarray) -> np.ndarray:
-    """Host (M, N) float64 image: sum of the ellipses of ``E`` on [-1, 1]^2."""
-    X, Y = np.meshgrid(np.linspace(-1, 1, N), -np.linspace(-1, 1, M))
+    """Host (M, N) float64 image: sum of the ellipses of ``E`` on [-1, 1]^2.
+
+    Each ellipse is tested only inside its bounding box (widened by two
+    grid steps); the per-pixel test is the same expression as on the full
+    grid, so the result is identical.
+    """
+    xs = np.linspace(-1, 1, N)
+    ys = -np.linspace(-1, 1, M)
     ph = np.zeros((M, N))
     ct, st = np.cos(E[:, 5]), np.sin(E[:, 5])
+    step_x = 2.0 / max(N - 1, 1)
+    step_y = 2.0 / max(M - 1, 1)
     for k in range(E.shape[0]):
         grey, a, b, xc, yc = E[k, 0], E[k, 1], E[k, 2], E[k, 3], E[k, 4]
+        hx = np.sqrt((a * ct[k]) ** 2 + (b * st[k]) ** 2) + 2 * step_x
+        hy = np.sqrt((a * st[k]) ** 2 + (b * ct[k]) ** 2) + 2 * step_y
+        cols = np.nonzero(np.abs(xs - xc) <= hx)[0]
+        rows = np.nonzero(np.abs(ys - yc) <= hy)[0]
+        if len(cols) == 0 or len(rows) == 0:
+            continue
+        c0, c1 = cols[0], cols[-1] + 1
+        r0, r1 = rows[0], rows[-1] + 1
+        X, Y = np.meshgrid(xs[c0:c1], ys[r0:r1])
         inside = ((X - xc) * ct[k] + (Y - yc) * st[k]) ** 2 / a**2 + (
             (X - xc) * st[k] - (Y - yc) * ct[k]
         ) ** 2 / b**2 <= 1
-        ph[inside] += grey
+        ph[r0:r1, c0:c1][inside] += grey
     return ph
 
 

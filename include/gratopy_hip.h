@@ -39,6 +39,16 @@ enum { GT_OK = 0, GT_EINVAL = 1, GT_ENOSPC = 2, GT_EHIP = 3 };
 /* Library version (major*10000 + minor*100 + patch). */
 int gt_version(void);
 
+/* Launch-shape knobs (process-wide; set by the autotuner, gratopy_amd/autotune.py).
+ * No reference counterpart: the reference leaves local sizes to the OpenCL
+ * driver (gratopy/gratopy.py:304-313).  Results do not depend on them beyond
+ * rounding.
+ *   GT_TUNE_FWD_GROUP: detector bins per lane in the forward projection (1, 2, 4, 8)
+ *   GT_TUNE_BACK_ROWS: pixels per lane in the backprojection (1, 2, 4) */
+enum { GT_TUNE_FWD_GROUP = 0, GT_TUNE_BACK_ROWS = 1 };
+int gt_tune_set(int key, int value);
+int gt_tune_get(int key);
+
 /* Copies the calling thread's last error message into buf (NUL terminated,
  * truncated to n bytes); returns the full message length. */
 size_t gt_last_error(char *buf, size_t n);

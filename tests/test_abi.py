@@ -1,0 +1,80 @@
+"""The C ABI of libgratopy_hip.so on the CPU: the library loads, exports
+every function include/gratopy_hip.h declares, matches the ctypes
+prototypes, and rejects invalid arguments with a status code and message
+before touching the device."""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from gratopy_amd import _hip
+from gratopy_amd import build as hip_build
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "gratopy_hip.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gt_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    hip_build.ensure_built()
+    return _hip.load_library()
+
+
+def test_header_declares_the_bound_prototypes():
+    assert _declared() == sorted(_hip.PROTOTYPES)
+
+
+def test_library_exports_every_declared_symbol(lib):
+    for name in _declared():
+        assert hasattr(lib, name), name
+
+
+def test_version_and_empty_error(lib):
+    assert lib.gt_version() >= 10000
+    buf = ctypes.create_string_buffer(8)
+    lib.gt_last_error(buf, 8)
+
+
+def test_invalid_arguments_are_rejected_without_a_device(lib):
+    # NULL arrays
+    rc = lib.gt_radon_fwd(None, 0, None, None, None, None, 4, 4, 4, 4, 1, 0, 0, None, 0)
+    assert rc == _hip.GT_EINVAL and "NULL" in _hip.last_error()
+    # bad dtype
+    rc = lib.gt_fan_back(None, 7, 1, 2, 3, 4, 5, 4, 4, 4, 4, 1, 0, 0, None, 0)
+    assert rc == _hip.GT_EINVAL and "dtype" in _hip.last_error()
+    # non-positive sizes
+    rc = lib.gt_radon_back(None, 0, 1, 2, 3, 4, 0, 4, 4, 4, 1, 0, 0, 8, 1 << 20)
+    assert rc == _hip.GT_EINVAL and "positive" in _hip.last_error()
+    # workspace too small
+    need = lib.gt_projection_workspace_bytes(0, 1, 0, 8, 8, 8, 8, 1, 0)
+    assert need > 0
+    rc = lib.gt_radon_back(None, 0, 1, 2, 3, 4, 8, 8, 8, 8, 1, 0, 0, 8, need - 1)
+    assert rc == _hip.GT_ENOSPC and "workspace" in _hip.last_error()
+    # aliasing output and input
+    rc = lib.gt_radon_fwd(None, 0, 5, 5, 3, 4, 4, 4, 4, 4, 1, 0, 0, 8, 1 << 20)
+    assert rc == _hip.GT_EINVAL and "alias" in _hip.last_error()
+    # pixel outside the image for the single-line kernel
+    rc = lib.gt_single_line(None, 0, 0, 1, 9, 0, 2, None, 3, 4, 4, 4, 4, 0)
+    assert rc == _hip.GT_EINVAL
+    with pytest.raises(ValueError, match="dtype"):
+        _hip.check(lib.gt_weight_sino(None, 5, 1, 2, 3, 4, 4, 1, 0, 0), "gt_weight_sino")
+
+
+def test_workspace_sizes(lib):
+    es = 4
+    # forward, F-ordered image: one transposed copy
+    assert lib.gt_projection_workspace_bytes(0, 0, 0, 100, 60, 80, 30, 1, 0) >= es * 100 * 60
+    # forward, C-ordered 3-D image: two copies
+    assert lib.gt_projection_workspace_bytes(1, 0, 0, 100, 60, 80, 30, 3, 1) >= 2 * es * 100 * 60 * 3
+    # backward: padded sinogram + per-angle table
+    assert lib.gt_projection_workspace_bytes(1, 1, 1, 100, 60, 80, 30, 2, 0) >= 8 * (84 * 30 * 2 + 8 * 30)
