@@ -1,0 +1,130 @@
+"""Launch-shape autotuner for the projection kernels.
+
+The forward kernels assign K adjacent detector bins to a lane (K in 1, 2,
+4, 8; K = 1 is the per-ray kernel) and the backprojection assigns RY pixels
+along y to a lane (RY in 1, 2, 4).  :func:`tune` times every variant on the
+caller's own geometry and array shapes with HIP events, keeps the fastest,
+and remembers it per shape key in a JSON cache
+(``$GRATOPY_AMD_TUNE_CACHE``, default ``~/.cache/gratopy_amd/tune.json``).
+:func:`apply` installs the cached choice for a shape before a run.  The
+variants compute the same sums (to rounding), so tuning never changes
+results beyond rounding.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+import threading
+
+import numpy as np
+
+from . import _device as dev
+from . import _hip
+
+FWD_GROUPS = (1, 2, 4, 8)
+BACK_ROWS = (1, 2, 4)
+KEY_FWD, KEY_BACK = 0, 1
+
+_lock = threading.Lock()
+
+
+def cache_path() -> str:
+    return os.environ.get(
+        "GRATOPY_AMD_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "gratopy_amd", "tune.json")
+    )
+
+
+def shape_key(projectionsetting, dtype, nz: int) -> str:
+    ps = projectionsetting
+    geom = "fan" if ps.is_fan else "radon"
+    Nx, Ny = ps.img_shape
+    Ns, Na = ps.sinogram_shape
+    return f"{geom}:{np.dtype(dtype).name}:{Nx}x{Ny}:{Ns}x{Na}:z{nz}"
+
+
+def _load() -> dict:
+    try:
+        with open(cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(table: dict) -> None:
+    path = cache_path()
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(table, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass  # a read-only home only loses the cache
+
+
+def set_variant(fwd_group: int | None = None, back_rows: int | None = None) -> None:
+    lib = _hip.load_library()
+    if fwd_group is not None:
+        _hip.check(lib.gt_tune_set(KEY_FWD, int(fwd_group)), "gt_tune_set")
+    if back_rows is not None:
+        _hip.check(lib.gt_tune_set(KEY_BACK, int(back_rows)), "gt_tune_set")
+
+
+def current_variant() -> dict:
+    lib = _hip.load_library()
+    return {"fwd_group": lib.gt_tune_get(KEY_FWD), "back_rows": lib.gt_tune_get(KEY_BACK)}
+
+
+def _time(fn, repeats: int) -> float:
+    torch = dev.torch
+    fn()  # warm-up (also compiles nothing: kernels are precompiled)
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    times = []
+    for _ in range(repeats):
+        start.record()
+        fn()
+        stop.record()
+        stop.synchronize()
+        times.append(start.elapsed_time(stop))
+    return float(np.median(times))
+
+
+def tune(projectionsetting, img, sino, repeats: int = 3, store: bool = True) -> dict:
+    """Time all variants on (img, sino) and install the fastest.
+
+    ``img`` and ``sino`` are device arrays of the shapes and orders the caller
+    will use; their contents are overwritten.
+    """
+    from .gratopy import backprojection, forwardprojection
+
+    ps = projectionsetting
+    fwd_times = {}
+    for k in FWD_GROUPS:
+        set_variant(fwd_group=k)
+        fwd_times[k] = _time(lambda: forwardprojection(img, ps, sino=sino), repeats)
+    back_times = {}
+    for r in BACK_ROWS:
+        set_variant(back_rows=r)
+        back_times[r] = _time(lambda: backprojection(sino, ps, img=img), repeats)
+    best = {"fwd_group": min(fwd_times, key=fwd_times.get), "back_rows": min(back_times, key=back_times.get),
+            "fwd_ms": fwd_times, "back_ms": back_times}
+    set_variant(best["fwd_group"], best["back_rows"])
+    if store:
+        nz = int(img.shape[2]) if img.dim() > 2 else 1
+        with _lock:
+            table = _load()
+            table[shape_key(ps, dev.to_np_dtype(img.dtype), nz)] = {
+                "fwd_group": best["fwd_group"], "back_rows": best["back_rows"]}
+            _store(table)
+    return best
+
+
+def apply(projectionsetting, dtype, nz: int = 1) -> bool:
+    """Install the cached variant for this shape; False when none is cached."""
+    entry = _load().get(shape_key(projectionsetting, dtype, nz))
+    if not entry:
+        return False
+    set_variant(entry.get("fwd_group"), entry.get("back_rows"))
+    return True

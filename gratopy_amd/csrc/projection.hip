@@ -546,8 +546,10 @@ __global__ __launch_bounds__(256) void stage_sino_kernel(real *__restrict__ S,
   S[((int64_t)z * Na + a) * Nsp + j] = v;
 }
 
-// One lane per pixel, 16x16 pixels per block.
-template <typename real, bool FAN>
+// One lane per RY pixels (x, y0..y0+RY-1); a block covers 16 x 16*RY pixels.
+// Per angle the lane evaluates the affine coordinates once and advances them
+// by one row step per pixel; the RY accumulators are independent chains.
+template <typename real, bool FAN, int RY>
 __global__ __launch_bounds__(256) void back_kernel(real *__restrict__ img, Strides3 io,
                                                    const real *__restrict__ S,
                                                    const AngleRec<real> *__restrict__ tab,
@@ -555,37 +557,52 @@ __global__ __launch_bounds__(256) void back_kernel(real *__restrict__ img, Strid
                                                    int Ns, int Na) {
   using A = Arith<real>;
   const int x = blockIdx.x * 16 + (threadIdx.x & 15);
-  const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  const int y0 = (blockIdx.y * 16 + (threadIdx.x >> 4)) * RY;
   const int z = blockIdx.z;
-  if (x >= Nx || y >= Ny) return;
-  const real xr = (real)x, yr = (real)y;
+  if (x >= Nx || y0 >= Ny) return;
+  const real xr = (real)x, yr = (real)y0;
   const int Nsp = Ns + 4;
   const real *Sz = S + (int64_t)z * Na * Nsp + 2;
   const real lo = (real)-1.5, hi = (real)Ns + (real)0.5;
   const real mid_det = FAN ? geo[5] : (real)0;
-  real acc = 0;
-#pragma unroll 4
+  real acc[RY];
+#pragma unroll
+  for (int r = 0; r < RY; ++r) acc[r] = 0;
+#pragma unroll 2
   for (int a = 0; a < Na; ++a) {
     const AngleRec<real> &c = tab[a];
-    real p, r = 1;
-    if (FAN) {
-      const real t = c.c[0] * xr + c.c[1] * yr + c.c[2];
-      const real ss = c.c[3] * xr + c.c[4] * yr + c.c[5];
-      r = A::rcp(t);
-      p = ss * r + mid_det;
-    } else {
-      p = c.c[0] * xr + c.c[1] * yr + c.c[2];
-    }
-    p = A::clamp(p, lo, hi);
-    const real fl = floor(p);
-    const int i = (int)fl;
-    const real f = p - fl;
     const real *row = Sz + (int64_t)a * Nsp;
-    const real v0 = row[i], v1 = row[i + 1];
-    const real v = v0 + f * (v1 - v0);
-    acc = FAN ? acc + v * r : acc + v;
+    real t = FAN ? c.c[0] * xr + c.c[1] * yr + c.c[2] : (real)1;
+    real q = FAN ? c.c[3] * xr + c.c[4] * yr + c.c[5] : c.c[0] * xr + c.c[1] * yr + c.c[2];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      real p, w = 1;
+      if (FAN) {
+        w = A::rcp(t);
+        p = q * w + mid_det;
+      } else {
+        p = q;
+      }
+      p = A::clamp(p, lo, hi);
+      const real fl = floor(p);
+      const int i = (int)fl;
+      const real f = p - fl;
+      const real v0 = row[i], v1 = row[i + 1];
+      const real v = v0 + f * (v1 - v0);
+      acc[r] = FAN ? acc[r] + v * w : acc[r] + v;
+      if (FAN) {
+        t += c.c[1];
+        q += c.c[4];
+      } else {
+        q += c.c[1];
+      }
+    }
   }
-  img[x * io.s0 + y * io.s1 + z * io.s2] = acc;
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    const int y = y0 + r;
+    if (y < Ny) img[x * io.s0 + y * io.s1 + z * io.s2] = acc[r];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -801,9 +818,17 @@ static int back_impl(hipStream_t st, real *img, const real *sino, const real *of
   hipLaunchKernelGGL((stage_sino_kernel<real, FAN>), dim3((L.Ns + 4 + 255) / 256, L.Na, L.Nz),
                      dim3(256), 0, st, S, sino, si, ofs, sdpd, geo, L.Ns, L.Na);
   const Strides3 io = order_strides(img_order, L.Nx, L.Ny, L.Nz);
-  dim3 grid((L.Nx + 15) / 16, (L.Ny + 15) / 16, L.Nz);
-  hipLaunchKernelGGL((back_kernel<real, FAN>), grid, dim3(256), 0, st, img, io, S, tab, geo, L.Nx,
-                     L.Ny, L.Ns, L.Na);
+  const int RY = tuning().back_rows;
+  dim3 grid((L.Nx + 15) / 16, (L.Ny + 16 * RY - 1) / (16 * RY), L.Nz);
+  if (RY == 4)
+    hipLaunchKernelGGL((back_kernel<real, FAN, 4>), grid, dim3(256), 0, st, img, io, S, tab, geo,
+                       L.Nx, L.Ny, L.Ns, L.Na);
+  else if (RY == 2)
+    hipLaunchKernelGGL((back_kernel<real, FAN, 2>), grid, dim3(256), 0, st, img, io, S, tab, geo,
+                       L.Nx, L.Ny, L.Ns, L.Na);
+  else
+    hipLaunchKernelGGL((back_kernel<real, FAN, 1>), grid, dim3(256), 0, st, img, io, S, tab, geo,
+                       L.Nx, L.Ny, L.Ns, L.Na);
   return check_launch(FAN ? "fan backprojection" : "radon backprojection");
 }
 

@@ -55,6 +55,8 @@ def _wait(stream, wait_for):
 
 
 def _record(stream):
+    if dev.torch.cuda.is_current_stream_capturing():
+        return None  # no event nodes inside a captured graph
     ev = dev.torch.cuda.Event()
     ev.record(stream)
     return ev
@@ -738,10 +740,43 @@ def normest(projectionsetting, number_iterations=50, dtype="float32", allocator=
     return np.sqrt(normsqr)
 
 
-def landweber(sino, projectionsetting, number_iterations=100, w=1.0, *, norm_estimate=None):
+def _iterate(bodies, number_iterations, projectionsetting, graph, label):
+    """Run ``bodies[i % len(bodies)](i)`` for i < number_iterations.
+
+    Eager mode enqueues every launch from Python, like the reference.  With
+    ``graph=True`` one period (all bodies in order) is captured once into a
+    HIP graph (on the settings' stream, or on torch's capture side stream
+    when the settings use the current stream) and replayed on the stream the
+    eager launches would use; leftover iterations run eagerly.
+    The captured launches are the eager ones, so results are identical.
+    Returns the number of iterations run.
+    """
+    torch = dev.torch
+    period = len(bodies)
+    n_graph = (number_iterations // period) if graph else 0
+    if n_graph:
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=projectionsetting.queue.stream):
+            for k, body in enumerate(bodies):
+                body(k)
+        with torch.cuda.stream(_stream_of(projectionsetting)):
+            for r in range(n_graph):
+                g.replay()
+                sys.stdout.write("\rProgress at {:3.0%}".format(float(r * period) / number_iterations))
+        del g
+    for i in range(n_graph * period, number_iterations):
+        sys.stdout.write("\rProgress at {:3.0%}".format(float(i) / number_iterations))
+        bodies[i % period](i)
+    print(f"\r{label} complete")
+    return number_iterations
+
+
+def landweber(sino, projectionsetting, number_iterations=100, w=1.0, *, norm_estimate=None,
+              graph=False):
     """Landweber (SIRT) iteration u <- u - w/||R||^2 R^#(R u - f), u0 = 0
-    (gratopy.py:2380-2446).  ``norm_estimate`` (keyword, not in the
-    reference) skips :func:`normest` when the operator norm is known."""
+    (gratopy.py:2380-2446).  Keywords not in the reference:
+    ``norm_estimate`` skips :func:`normest` when the operator norm is known;
+    ``graph=True`` replays the iteration as a captured HIP graph."""
     ps = projectionsetting
     _require_device_array(sino, "sino")
     order = "C" if dev.is_c_contiguous(sino) else "F"
@@ -750,28 +785,34 @@ def landweber(sino, projectionsetting, number_iterations=100, w=1.0, *, norm_est
     dtype = dev.to_np_dtype(sino.dtype)
     w = dtype.type(w / norm_estimate**2)
     stream = _stream_of(ps)
+    ps.ensure_dtype(dtype)
     with dev.torch.cuda.stream(stream):
         sinonew = sino.clone(memory_format=dev.torch.preserve_format)
         z = (int(sino.shape[2]),) if sino.dim() > 2 else ()
         U = dev.empty(ps.img_shape + z, dtype, order=order, device=sino.device)
         Unew = dev.zeros(ps.img_shape + z, dtype, order=order, device=sino.device)
     dcode = _hip.dtype_code(dtype)
-    for i in range(number_iterations):
-        sys.stdout.write("\rProgress at {:3.0%}".format(float(i) / number_iterations))
+
+    def body(_):
+        s = _stream_of(ps)
         forwardprojection(Unew, ps, sino=sinonew)
-        _hip.call("gt_sub", stream.cuda_stream, dcode, sinonew.data_ptr(), sinonew.data_ptr(),
+        _hip.call("gt_sub", s.cuda_stream, dcode, sinonew.data_ptr(), sinonew.data_ptr(),
                   sino.data_ptr(), sino.numel())
         backprojection(sinonew, ps, img=U)
-        _hip.call("gt_axpy", stream.cuda_stream, dcode, Unew.data_ptr(), float(-w), U.data_ptr(),
+        _hip.call("gt_axpy", s.cuda_stream, dcode, Unew.data_ptr(), float(-w), U.data_ptr(),
                   U.numel())
-    print("\rLandweber reconstruction complete")
+
+    _iterate([body], number_iterations, ps, graph, "Landweber reconstruction")
     return Unew
 
 
-def conjugate_gradients(sino, projectionsetting, number_iterations=20, epsilon=0.0, x0=None):
+def conjugate_gradients(sino, projectionsetting, number_iterations=20, epsilon=0.0, x0=None, *,
+                        graph=False):
     """Conjugate gradients on the normal equations in the weighted spaces
     (gratopy.py:2449-2551).  alpha and beta are formed on the device; the
-    host reads the residual only when ``epsilon > 0`` (early stopping)."""
+    host reads the residual only when ``epsilon > 0`` (early stopping).
+    ``graph=True`` (keyword, not in the reference; ignored when
+    ``epsilon > 0``) replays the iterations as a captured HIP graph."""
     ps = projectionsetting
     _require_device_array(sino, "sino")
     torch = dev.torch
@@ -779,6 +820,7 @@ def conjugate_gradients(sino, projectionsetting, number_iterations=20, epsilon=0
     order = "C" if dev.is_c_contiguous(sino) else "F"
     dtype = dev.to_np_dtype(sino.dtype)
     stream = _stream_of(ps)
+    ps.ensure_dtype(dtype)
     with torch.cuda.stream(stream):
         if x0 is None:
             x0 = dev.zeros(dims, dtype, order=order, device=sino.device)
@@ -803,23 +845,30 @@ def conjugate_gradients(sino, projectionsetting, number_iterations=20, epsilon=0
     red.dot(stream, 0, sold, sold)
     if epsilon > 0:
         red.dot(stream, 6, sino, sino)
-    for k in range(number_iterations):
-        sys.stdout.write("\rProgress at {:3.0%}".format(float(k) / number_iterations))
+    buffers = [sold, snew]
+
+    def body(k):
+        # buffers[k % 2] holds s_old on entry; the new gradient goes to the other one
+        s = _stream_of(ps)
+        s_old, s_new = buffers[k % 2], buffers[(k + 1) % 2]
         forwardprojection(p, ps, sino=q)
         weight_sinogram(q, ps, q_rescaled)
-        red.dot(stream, 1, q_rescaled, q)
-        _hip.call("gt_cg_scalars", stream.cuda_stream, red.slot(0), float(scale))
-        _hip.call("gt_axpy_dev", stream.cuda_stream, dcode, x.data_ptr(), red.slot(4), 1.0,
+        red.dot(s, 1, q_rescaled, q)
+        _hip.call("gt_cg_scalars", s.cuda_stream, red.slot(0), float(scale))
+        _hip.call("gt_axpy_dev", s.cuda_stream, dcode, x.data_ptr(), red.slot(4), 1.0,
                   p.data_ptr(), n_img)
-        _hip.call("gt_axpy_dev", stream.cuda_stream, dcode, d.data_ptr(), red.slot(4), -1.0,
+        _hip.call("gt_axpy_dev", s.cuda_stream, dcode, d.data_ptr(), red.slot(4), -1.0,
                   q.data_ptr(), n_sino)
-        backprojection(d, ps, img=snew)
-        red.dot(stream, 2, snew, snew)
-        _hip.call("gt_cg_scalars", stream.cuda_stream, red.slot(0), 0.0)
-        sold, snew = snew, sold
-        _hip.call("gt_axpby_dev", stream.cuda_stream, dcode, p.data_ptr(), red.slot(5), 1.0,
-                  p.data_ptr(), sold.data_ptr(), n_img)
-        if epsilon > 0:
+        backprojection(d, ps, img=s_new)
+        red.dot(s, 2, s_new, s_new)
+        _hip.call("gt_cg_scalars", s.cuda_stream, red.slot(0), 0.0)
+        _hip.call("gt_axpby_dev", s.cuda_stream, dcode, p.data_ptr(), red.slot(5), 1.0,
+                  p.data_ptr(), s_new.data_ptr(), n_img)
+
+    if epsilon > 0:
+        for k in range(number_iterations):
+            sys.stdout.write("\rProgress at {:3.0%}".format(float(k) / number_iterations))
+            body(k)
             red.dot(stream, 3, d, d)
             stream.synchronize()
             vals = red.values.cpu().numpy()
@@ -827,12 +876,14 @@ def conjugate_gradients(sino, projectionsetting, number_iterations=20, epsilon=0
             if residue < epsilon:
                 print("\rProgress aborted prematurely as desired" + "precision is reached")
                 break
-    print("\rCG reconstruction complete")
+        print("\rCG reconstruction complete")
+    else:
+        _iterate([body, body], number_iterations, ps, graph, "CG reconstruction")
     return x
 
 
 def total_variation(sino, projectionsetting, mu, number_iterations=1000, slice_thickness=1.0,
-                    stepsize_weighting=10.0, *, norm_estimate=None):
+                    stepsize_weighting=10.0, *, norm_estimate=None, graph=False):
     """Chambolle-Pock primal-dual iteration for
     min_u mu/2 ||R u - f||^2 + TV(u), u >= 0 (gratopy.py:2554-2762).
 
@@ -841,8 +892,9 @@ def total_variation(sino, projectionsetting, mu, number_iterations=1000, slice_t
     primal + extragradient update.  A 3-D sinogram is reconstructed slice by
     slice coupled through ``slice_thickness`` (the reference raises
     TypeError on 3-D input at gratopy.py:2626-2628); ``slice_thickness=0``
-    decouples the slices.  ``norm_estimate`` (keyword, not in the reference)
-    skips :func:`normest`.
+    decouples the slices.  Keywords not in the reference: ``norm_estimate``
+    skips :func:`normest`; ``graph=True`` replays the iterations as a
+    captured HIP graph.
     """
     ps = projectionsetting
     _require_device_array(sino, "sino")
@@ -877,17 +929,24 @@ def total_variation(sino, projectionsetting, mu, number_iterations=1000, slice_t
     dcode = _hip.dtype_code(dtype)
     ocode = dev.order_code(order)
     Ns, Na = ps.sinogram_shape
-    for i in range(number_iterations):
-        _hip.call("gt_tv_dual", stream.cuda_stream, dcode, V.data_ptr(), U_.data_ptr(),
+    ps.ensure_dtype(dtype)
+    pair = [U, U_]
+
+    def body(i):
+        # pair[i % 2] is the primal iterate u, pair[(i+1) % 2] the extrapolation u_bar
+        s = _stream_of(ps)
+        u, u_bar = pair[i % 2], pair[(i + 1) % 2]
+        _hip.call("gt_tv_dual", s.cuda_stream, dcode, V.data_ptr(), u_bar.data_ptr(),
                   float(sigma), float(slice_thickness), Nx, Ny, Nz, ocode)
-        forwardprojection(U_, ps, sino=KU)
-        _hip.call("gt_tv_update_lambda", stream.cuda_stream, dcode, Lamb.data_ptr(), KU.data_ptr(),
+        forwardprojection(u_bar, ps, sino=KU)
+        _hip.call("gt_tv_update_lambda", s.cuda_stream, dcode, Lamb.data_ptr(), KU.data_ptr(),
                   sino.data_ptr(), float(sigma / norm_estimate), float(mu), Ns, Na, Nz, ocode)
         backprojection(Lamb, ps, img=KSTAR)
-        _hip.call("gt_tv_primal", stream.cuda_stream, dcode, U_.data_ptr(), U.data_ptr(),
+        # u_bar <- max(0, u + tau(div v - K*lambda/||R||)) (the new u); u <- 2 u_new - u
+        _hip.call("gt_tv_primal", s.cuda_stream, dcode, u_bar.data_ptr(), u.data_ptr(),
                   V.data_ptr(), KSTAR.data_ptr(), float(tau), float(1.0 / norm_estimate),
                   float(slice_thickness), Nx, Ny, Nz, ocode)
-        U, U_ = U_, U
-        sys.stdout.write("\rProgress at {:3.0%}".format(float(i) / number_iterations))
-    print("\rTV reconstruction complete")
-    return U
+
+    n = _iterate([body, body], number_iterations, ps, graph, "TV reconstruction")
+    # after iteration i the new u sits in pair[(i+1) % 2]
+    return pair[n % 2]
