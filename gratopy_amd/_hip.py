@@ -69,6 +69,10 @@ PROTOTYPES: dict[str, tuple] = {
     "gt_tv_update_extra": (_i, [_vp, _i, _vp, _vp, _sz]),
     "gt_tv_dual": (_i, [_vp, _i, _vp, _vp, _f, _f, _i, _i, _i, _i]),
     "gt_tv_primal": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _f, _f, _f, _i, _i, _i, _i]),
+    "gt_tv_adj_primal": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _sz]),
+    "gt_tv_adj_dual": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _f, _f, _f, _i, _i, _i, _i]),
+    "gt_tv_adj_div": (_i, [_vp, _i, _vp, _vp, _d, _vp, _f, _f, _i, _i, _i, _i]),
+    "gt_tv_adj_lambda": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _d, _f, _f, _i, _i, _i, _i]),
 }
 
 _lib = None

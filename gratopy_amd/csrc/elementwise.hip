@@ -262,6 +262,117 @@ __global__ __launch_bounds__(256) void tv_extra_kernel(const real *u_, real *u, 
   GRID_STRIDE(i, n) u[i] = (real)2 * u_[i] - u[i];
 }
 
+// ---- adjoint (vector-Jacobian product) of one TV primal-dual step ------------
+// Forward step (as the kernels above implement it):
+//   a   = v + sigma grad(ubar)          v'  = a / max(1, |a|)
+//   lam'= (lam + sigma_p (K ubar - f)) c
+//   u'  = max(0, u + tau (div v' - n B lam'))        ubar' = 2 u' - u
+// The adjoint kernels below take the cotangents of (u', ubar', v', lam').
+// grad has forward differences with a zero last difference (Neumann), and
+// div = -grad^T for this pair (total_variation.cl:54-148).
+
+// wb = [u' > 0] (ub' + 2 ubarb');  ub = wb - ubarb'
+template <typename real>
+__global__ __launch_bounds__(256) void tv_adj_primal_kernel(real *wb, real *ub, const real *ub_next,
+                                                            const real *ubarb_next,
+                                                            const real *u_next, size_t n) {
+  GRID_STRIDE(i, n) {
+    const real g = ub_next[i] + (real)2 * ubarb_next[i];
+    const real w = u_next[i] > (real)0 ? g : (real)0;
+    const real ubar = ubarb_next[i];
+    wb[i] = w;
+    ub[i] = w - ubar;
+  }
+}
+
+template <typename real>
+__device__ inline void forward_grad(const real *__restrict__ f, const ImgIdx &p, int Nx, int Ny,
+                                    int Nz, Strides3 st, float zdist, real &g0, real &g1, real &g2) {
+  const real c = f[p.i];
+  g0 = p.x < Nx - 1 ? -c + f[p.i + st.s0] : (real)0;
+  g1 = p.y < Ny - 1 ? -c + f[p.i + st.s1] : (real)0;
+  g2 = p.z < Nz - 1 ? -c + f[p.i + st.s2] : (real)0;
+  g2 = zdist > 0.0f ? g2 / (real)zdist : (real)0;
+}
+
+// ab = J_P(a) (vb' - tau grad(wb)) with a = v + sigma grad(ubar) recomputed;
+// J_P = I where |a| <= 1, (I - a a^T/|a|^2)/|a| where the ball projection was active.
+template <typename real>
+__global__ __launch_bounds__(256) void tv_adj_dual_kernel(real *ab, const real *vb_next,
+                                                          const real *wb, const real *v,
+                                                          const real *ubar, float sigma, float tau,
+                                                          float zdist, int Nx, int Ny, int Nz,
+                                                          int order, Strides3 st) {
+  const size_t n = (size_t)Nx * Ny * Nz;
+  GRID_STRIDE(lin, n) {
+    const ImgIdx p = img_index(lin, Nx, Ny, order, st);
+    real g0, g1, g2, w0, w1, w2;
+    forward_grad(ubar, p, Nx, Ny, Nz, st, zdist, g0, g1, g2);
+    forward_grad(wb, p, Nx, Ny, Nz, st, zdist, w0, w1, w2);
+    const real *vp = v + 4 * p.i;
+    const real sg = (real)sigma, ta = (real)tau;
+    const real a0 = vp[0] + sg * g0, a1 = vp[1] + sg * g1, a2 = vp[2] + sg * g2;
+    const real *bp = vb_next + 4 * p.i;
+    real x0 = bp[0] - ta * w0, x1 = bp[1] - ta * w1, x2 = bp[2] - ta * w2;
+    const real norm = hypot(hypot(a0, a1), a2);
+    if (norm > (real)1) {
+      const real h0 = a0 / norm, h1 = a1 / norm, h2 = a2 / norm;
+      const real d = h0 * x0 + h1 * x1 + h2 * x2;
+      x0 = (x0 - h0 * d) / norm;
+      x1 = (x1 - h1 * d) / norm;
+      x2 = (x2 - h2 * d) / norm;
+    }
+    real *out = ab + 4 * p.i;
+    out[0] = x0;
+    out[1] = x1;
+    out[2] = x2;
+    out[3] = 0;
+  }
+}
+
+// ubarb = coef * bterm - sigma div(ab)
+template <typename real>
+__global__ __launch_bounds__(256) void tv_adj_div_kernel(real *ubarb, const real *bterm, real coef,
+                                                         const real *ab, float sigma, float zdist,
+                                                         int Nx, int Ny, int Nz, int order,
+                                                         Strides3 st) {
+  const size_t n = (size_t)Nx * Ny * Nz;
+  GRID_STRIDE(lin, n) {
+    const ImgIdx p = img_index(lin, Nx, Ny, order, st);
+    const real *vp = ab + 4 * p.i;
+    real d0 = vp[0], d1 = vp[1], d2 = vp[2];
+    if (p.x == Nx - 1) d0 = 0;
+    if (p.x > 0) d0 -= ab[4 * (p.i - st.s0) + 0];
+    if (p.y == Ny - 1) d1 = 0;
+    if (p.y > 0) d1 -= ab[4 * (p.i - st.s1) + 1];
+    if (p.z == Nz - 1) d2 = 0;
+    if (p.z > 0) d2 -= ab[4 * (p.i - st.s2) + 2];
+    d2 = zdist > 0.0f ? d2 / (real)zdist : (real)0;
+    ubarb[p.i] = coef * bterm[p.i] - (real)sigma * (d0 + d1 + d2);
+  }
+}
+
+// bb = c (lamb' - coefR * w_a * Rwb);  lamb = bb;  fb -= sigma_p * bb;  winv = bb / w_a
+template <typename real>
+__global__ __launch_bounds__(256) void tv_adj_lambda_kernel(real *lamb, real *fb, real *winv,
+                                                            const real *lamb_next, const real *rwb,
+                                                            const real *__restrict__ w, real coefR,
+                                                            float c, float sigma_p, int Ns, int Na,
+                                                            int Nz, Strides3 so) {
+  const size_t n = (size_t)Ns * Na * Nz;
+  GRID_STRIDE(i, n) {
+    const int s = (int)(i % Ns);
+    const size_t r = i / Ns;
+    const int a = (int)(r % Na);
+    const int z = (int)(r / Na);
+    const int64_t k = s * so.s0 + a * so.s1 + z * so.s2;
+    const real bb = (real)c * (lamb_next[k] - coefR * w[a] * rwb[k]);
+    lamb[k] = bb;
+    fb[k] -= (real)sigma_p * bb;
+    winv[k] = bb / w[a];
+  }
+}
+
 static int check_dtype(int dtype) {
   GT_REQUIRE(dtype == GT_FLOAT32 || dtype == GT_FLOAT64, "dtype must be GT_FLOAT32 or GT_FLOAT64");
   return GT_OK;
@@ -587,4 +698,90 @@ extern "C" int gt_tv_update_extra(gt_stream_t stream, int dtype, void *u_, void 
     hipLaunchKernelGGL(tv_extra_kernel<float>, stream_grid(n), dim3(256), 0, st,
                        AS(const float *, u_), AS(float *, u), n);
   return check_launch("tv_update_extra");
+}
+
+// ---- C ABI of the TV step adjoint -------------------------------------------
+extern "C" int gt_tv_adj_primal(gt_stream_t stream, int dtype, void *wb, void *ub,
+                                const void *ub_next, const void *ubarb_next, const void *u_next,
+                                size_t n) {
+  int rc = check_dtype(dtype);
+  if (rc) return rc;
+  GT_REQUIRE(wb && ub && ub_next && ubarb_next && u_next, "NULL array argument");
+  if (n == 0) return GT_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == GT_FLOAT64)
+    hipLaunchKernelGGL(tv_adj_primal_kernel<double>, stream_grid(n), dim3(256), 0, st,
+                       AS(double *, wb), AS(double *, ub), AS(const double *, ub_next),
+                       AS(const double *, ubarb_next), AS(const double *, u_next), n);
+  else
+    hipLaunchKernelGGL(tv_adj_primal_kernel<float>, stream_grid(n), dim3(256), 0, st,
+                       AS(float *, wb), AS(float *, ub), AS(const float *, ub_next),
+                       AS(const float *, ubarb_next), AS(const float *, u_next), n);
+  return check_launch("tv_adj_primal");
+}
+
+extern "C" int gt_tv_adj_dual(gt_stream_t stream, int dtype, void *ab, const void *vb_next,
+                              const void *wb, const void *v, const void *ubar, float sigma,
+                              float tau, float z_distance, int Nx, int Ny, int Nz, int order) {
+  int rc = check_dtype(dtype);
+  if (rc || (rc = check_img(Nx, Ny, Nz, order))) return rc;
+  GT_REQUIRE(ab && vb_next && wb && v && ubar, "NULL array argument");
+  GT_REQUIRE(ab != vb_next && ab != v, "ab must not alias its inputs");
+  const Strides3 st3 = order_strides(order, Nx, Ny, Nz);
+  const size_t n = (size_t)Nx * Ny * Nz;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == GT_FLOAT64)
+    hipLaunchKernelGGL(tv_adj_dual_kernel<double>, stream_grid(n), dim3(256), 0, st,
+                       AS(double *, ab), AS(const double *, vb_next), AS(const double *, wb),
+                       AS(const double *, v), AS(const double *, ubar), sigma, tau, z_distance, Nx,
+                       Ny, Nz, order, st3);
+  else
+    hipLaunchKernelGGL(tv_adj_dual_kernel<float>, stream_grid(n), dim3(256), 0, st,
+                       AS(float *, ab), AS(const float *, vb_next), AS(const float *, wb),
+                       AS(const float *, v), AS(const float *, ubar), sigma, tau, z_distance, Nx,
+                       Ny, Nz, order, st3);
+  return check_launch("tv_adj_dual");
+}
+
+extern "C" int gt_tv_adj_div(gt_stream_t stream, int dtype, void *ubarb, const void *bterm,
+                             double coef, const void *ab, float sigma, float z_distance, int Nx,
+                             int Ny, int Nz, int order) {
+  int rc = check_dtype(dtype);
+  if (rc || (rc = check_img(Nx, Ny, Nz, order))) return rc;
+  GT_REQUIRE(ubarb && bterm && ab, "NULL array argument");
+  const Strides3 st3 = order_strides(order, Nx, Ny, Nz);
+  const size_t n = (size_t)Nx * Ny * Nz;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == GT_FLOAT64)
+    hipLaunchKernelGGL(tv_adj_div_kernel<double>, stream_grid(n), dim3(256), 0, st,
+                       AS(double *, ubarb), AS(const double *, bterm), coef, AS(const double *, ab),
+                       sigma, z_distance, Nx, Ny, Nz, order, st3);
+  else
+    hipLaunchKernelGGL(tv_adj_div_kernel<float>, stream_grid(n), dim3(256), 0, st,
+                       AS(float *, ubarb), AS(const float *, bterm), (float)coef,
+                       AS(const float *, ab), sigma, z_distance, Nx, Ny, Nz, order, st3);
+  return check_launch("tv_adj_div");
+}
+
+extern "C" int gt_tv_adj_lambda(gt_stream_t stream, int dtype, void *lamb, void *fb, void *winv,
+                                const void *lamb_next, const void *rwb, const void *angle_weights,
+                                double coef_r, float c, float sigma_p, int Ns, int Na, int Nz,
+                                int order) {
+  int rc = check_dtype(dtype);
+  if (rc || (rc = check_img(Ns, Na, Nz, order))) return rc;
+  GT_REQUIRE(lamb && fb && winv && lamb_next && rwb && angle_weights, "NULL array argument");
+  const Strides3 so = order_strides(order, Ns, Na, Nz);
+  const size_t n = (size_t)Ns * Na * Nz;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == GT_FLOAT64)
+    hipLaunchKernelGGL(tv_adj_lambda_kernel<double>, stream_grid(n), dim3(256), 0, st,
+                       AS(double *, lamb), AS(double *, fb), AS(double *, winv),
+                       AS(const double *, lamb_next), AS(const double *, rwb),
+                       AS(const double *, angle_weights), coef_r, c, sigma_p, Ns, Na, Nz, so);
+  else
+    hipLaunchKernelGGL(tv_adj_lambda_kernel<float>, stream_grid(n), dim3(256), 0, st,
+                       AS(float *, lamb), AS(float *, fb), AS(float *, winv),
+                       AS(const float *, lamb_next), AS(const float *, rwb),
+                       AS(const float *, angle_weights), (float)coef_r, c, sigma_p, Ns, Na, Nz, so);
+  return check_launch("tv_adj_lambda");
 }

@@ -1,0 +1,566 @@
+// Device bodies of the projection kernels, shared by the single-problem
+// launches (projection.hip) and the batched persistent launches (batch.hip).
+//
+// Forward (gratopy/radon.cl:66-161, gratopy/fanbeam.cl:72-256):
+//   sino[s,a,z] = sum over the pixels of the strip between the rays through
+//   detector positions s-1 and s+1 of hat-weight * img.  Rays closer to the
+//   x axis walk a y-fastest copy of the image (made by a tiled transpose), so
+//   a wave always reads along the fastest axis; rows that cannot touch the
+//   image are skipped (closed-form row range).
+//   * per-ray body: one lane per detector bin, the reference's loop order;
+//   * bin-group body: a lane owns K adjacent bins [s0, s0+K) and walks the
+//     union strip between the rays through s0-1 and s0+K once per row; each
+//     strip pixel is projected once and its two hat weights (1-f, f) go to
+//     bins floor(p), floor(p)+1, accumulated in lane-private LDS slots.  Per
+//     bin the summation order is the reference's (rows ascending, then along
+//     the row), so results agree to rounding.
+// Backward (gratopy/radon.cl:183-241, gratopy/fanbeam.cl:290-391):
+//   img[x,y,z] = sum_a lerp(S_a, p_a(x,y)) [ * 1/t_a(x,y) for fanbeam ] over a
+//   zero-padded sinogram S already multiplied by the angle weight (and
+//   sdpd[s]/R for fanbeam) and a per-angle affine table read with scalar loads.
+#pragma once
+
+#include "common.h"
+
+namespace gt {
+
+template <typename real>
+struct alignas(sizeof(real) * 8) AngleRec {
+  real c[8];
+};
+
+// Everything one projection needs.  Forward jobs use sino/so/imgX/imgY,
+// backward jobs img/io/S/tab.  Epilogues: forward mode 1 stores
+// value - data[k] (data shares the output's strides); backward mode 1 stores
+// img[k] + scale * value.
+template <typename real>
+struct Job {
+  int fan;
+  int Nx, Ny, Ns, Na, Nz;
+  const real *ofs, *sdpd, *geo;
+  real *sino;
+  Strides3 so;
+  const real *imgX, *imgY;
+  real *img;
+  Strides3 io;
+  const real *S;
+  const AngleRec<real> *tab;
+  int mode;
+  const real *data;
+  real scale;
+};
+
+template <int K>
+struct Slots {
+  static constexpr int kCount = K + 4;
+};
+
+// ---------------------------------------------------------------------------
+// epilogues
+// ---------------------------------------------------------------------------
+template <typename real>
+__device__ __forceinline__ void store_fwd(const Job<real> &J, int s, int a, int z, real v) {
+  const int64_t k = s * J.so.s0 + a * J.so.s1 + z * J.so.s2;
+  if (J.mode == 1) v -= J.data[k];
+  J.sino[k] = v;
+}
+
+template <typename real>
+__device__ __forceinline__ void store_back(const Job<real> &J, int x, int y, int z, real v) {
+  const int64_t k = x * J.io.s0 + y * J.io.s1 + z * J.io.s2;
+  if (J.mode == 1) v = J.img[k] + J.scale * v;
+  J.img[k] = v;
+}
+
+// ---------------------------------------------------------------------------
+// 32x32 tile of a copy between two orders (LDS transpose).
+// ---------------------------------------------------------------------------
+template <typename real>
+__device__ __forceinline__ void relayout_tile(real (*tile)[33], real *__restrict__ dst, Strides3 ds,
+                                              const real *__restrict__ src, Strides3 ss, int N0,
+                                              int N1, int64_t z, int b0, int b1) {
+  const int read0fast = ss.s0 <= ss.s1;
+  const int write0fast = ds.s0 <= ds.s1;
+  const int lx = threadIdx.x & 31, ly = threadIdx.x >> 5;
+  for (int k = ly; k < 32; k += 8) {
+    const int i0 = read0fast ? b0 + lx : b0 + k;
+    const int i1 = read0fast ? b1 + k : b1 + lx;
+    if (i0 < N0 && i1 < N1) tile[i0 - b0][i1 - b1] = src[i0 * ss.s0 + i1 * ss.s1 + z * ss.s2];
+  }
+  __syncthreads();
+  for (int k = ly; k < 32; k += 8) {
+    const int i0 = write0fast ? b0 + lx : b0 + k;
+    const int i1 = write0fast ? b1 + k : b1 + lx;
+    if (i0 < N0 && i1 < N1) dst[i0 * ds.s0 + i1 * ds.s1 + z * ds.s2] = tile[i0 - b0][i1 - b1];
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Row ranges.  Conservative [ylo, yhi]: rows outside have no pixel in the
+// ray's strip (one row and one column of margin).
+// ---------------------------------------------------------------------------
+__device__ inline void clip_rows(double lo, double hi, int Nyy, int &ylo, int &yhi) {
+  if (lo != lo || hi != hi) {  // NaN: fall back to every row
+    ylo = 0;
+    yhi = Nyy - 1;
+    return;
+  }
+  // clamp in floating point first: infinities must not reach the int conversion
+  lo = fmin(fmax(lo - 1.0, -1.0), (double)Nyy);
+  hi = fmax(fmin(hi + 1.0, (double)Nyy), -1.0);
+  ylo = (int)floor(lo);
+  yhi = (int)ceil(hi);
+  ylo = ylo < 0 ? 0 : ylo;
+  yhi = yhi > Nyy - 1 ? Nyy - 1 : yhi;
+}
+
+// m*u >= L restricts u to a half line (or to nothing / everything).
+__device__ inline void restrict_ge(double m, double L, double &lo, double &hi) {
+  if (m > 0)
+    lo = fmax(lo, L / m);
+  else if (m < 0)
+    hi = fmin(hi, L / m);
+  else if (L > 0) {
+    lo = 1.0;
+    hi = 0.0;
+  }
+}
+
+// Parallel strip of half width `half` detector units around detector
+// position c (axes already swapped): pixels with |x*X + y*Y + off - c| < half.
+// The strip centre in x is g(y) = -(off - c + y*Y)/X, half width half/|X|.
+__device__ inline void radon_rows(double X, double Y, double off_minus_c, double half, int Nxx,
+                                  int Nyy, int &ylo, int &yhi) {
+  const double h = half / fabs(X);
+  const double g0 = -off_minus_c / X, m = -Y / X;
+  double lo = -INFINITY, hi = INFINITY;
+  restrict_ge(m, -1.0 - h - g0, lo, hi);           // g(y) >= -1 - h
+  restrict_ge(-m, g0 - ((double)Nxx + h), lo, hi);  // g(y) <= Nxx + h
+  if (lo > hi) {
+    ylo = 0;
+    yhi = -1;
+    return;
+  }
+  clip_rows(lo, hi, Nyy, ylo, yhi);
+}
+
+// Fan strip (axes already swapped): row y is visited on [xlow(y), xhigh(y)]
+// with xhigh = qx + dpx*u, xlow = qx + dmx*u, u = y - midy - qy.
+__device__ inline void fan_rows(double qx, double qy, double dpx, double dmx, double midx,
+                                double midy, int Nxx, int Nyy, int &ylo, int &yhi) {
+  double lo = -INFINITY, hi = INFINITY;
+  restrict_ge(dpx, -midx - 1.0 - qx, lo, hi);              // xhigh >= -midx - 1
+  restrict_ge(-dmx, -((double)Nxx - midx - qx), lo, hi);  // xlow <= Nxx - midx
+  if (lo > hi) {
+    ylo = 0;
+    yhi = -1;
+    return;
+  }
+  clip_rows(lo + midy + qy, hi + midy + qy, Nyy, ylo, yhi);
+}
+
+// ---------------------------------------------------------------------------
+// Fan ray geometry of a strip bounded by the rays through detector positions
+// s_lo and s_hi (fanbeam.cl:100-184 with s-1, s+1 replaced by s_lo, s_hi).
+// ---------------------------------------------------------------------------
+template <typename real>
+struct FanStrip {
+  real ddx, ddy, dx, dy, qx, qy, dpx, dmx, midx, midy, RE;
+  int Nxx, Nyy;
+  const real *base;
+  int64_t row_stride;
+  // both bounding rays advance the same way along the row axis, so each row
+  // meets the strip in one interval [xlow(y), xhigh(y)].  The rays of a
+  // single bin always do on the reference's geometries; the union strip of a
+  // bin group can fail it when bins are very wide (a few detector bins over
+  // a large angle), and the group then projects its bins one by one.
+  bool regular;
+};
+
+template <typename real>
+__device__ __forceinline__ FanStrip<real> fan_strip(const Job<real> &J, int a, int z, real s_lo,
+                                                    real s_hi) {
+  FanStrip<real> F;
+  const real *geo = J.geo;
+  const real R = geo[0];
+  const real delta_xi = geo[2];
+  const real mid_det = geo[5];
+  F.midx = geo[3];
+  F.midy = geo[4];
+  F.RE = geo[1];
+  const real *o = J.ofs + 8 * (int64_t)a;
+  real dx = o[0], dy = o[1], qx = o[2], qy = o[3];
+  const real d0x = o[4], d0y = o[5];
+  const real sp = -mid_det + s_hi;
+  const real sm = -mid_det + s_lo;
+  real dpx = d0x + dx * sp - qx, dpy = d0y + dy * sp - qy;
+  real dmx = d0x + dx * sm - qx, dmy = d0y + dy * sm - qy;
+  real ddx = d0x - qx, ddy = d0y - qy;
+  const real RR = R * R;
+  ddx /= RR;
+  ddy /= RR;
+  const real dxi2 = delta_xi * delta_xi;
+  dx /= dxi2;
+  dy /= dxi2;
+  F.RE *= (real)1 / R;
+  F.Nxx = J.Nx;
+  F.Nyy = J.Ny;
+  const int64_t slice = (int64_t)z * J.Nx * J.Ny;
+  if (fabs(dpx) < fabs(dpy) && fabs(dmx) < fabs(dmy)) {
+    F.base = J.imgX + slice;
+    F.row_stride = J.Nx;
+  } else {
+    real t;
+    t = ddx; ddx = ddy; ddy = t;
+    t = dx; dx = dy; dy = t;
+    t = qx; qx = qy; qy = t;
+    t = dpx; dpx = dpy; dpy = t;
+    t = dmx; dmx = dmy; dmy = t;
+    t = F.midx; F.midx = F.midy; F.midy = t;
+    F.Nyy = J.Nx;
+    F.Nxx = J.Ny;
+    F.base = J.imgY + slice;
+    F.row_stride = J.Ny;
+  }
+  F.regular = dpy * dmy > (real)0;
+  dpx /= dpy;
+  dmx /= dmy;
+  if (qy * (dmx - dpx) < (real)0) {
+    const real t = dpx;
+    dpx = dmx;
+    dmx = t;
+  }
+  F.ddx = ddx; F.ddy = ddy; F.dx = dx; F.dy = dy; F.qx = qx; F.qy = qy;
+  F.dpx = dpx; F.dmx = dmx;
+  return F;
+}
+
+// ---------------------------------------------------------------------------
+// Forward bodies.
+// ---------------------------------------------------------------------------
+template <typename real>
+__device__ __forceinline__ void radon_fwd_ray(const Job<real> &J, int s, int a, int z) {
+  // the reference reads the scales as float even in double kernels (radon.cl:82-83)
+  const float delta_x = (float)J.geo[0];
+  const float delta_xi = (float)J.geo[1];
+  const real *o = J.ofs + 8 * (int64_t)a;
+  real X = o[0], Y = o[1];
+  const real off = o[2], inv = o[3];
+  int Nxx = J.Nx, Nyy = J.Ny;
+  const real *base;
+  int64_t row_stride;
+  const int64_t slice = (int64_t)z * J.Nx * J.Ny;
+  if (o[5] != (real)0) {
+    const real t = X;
+    X = Y;
+    Y = t;
+    Nxx = J.Ny;
+    Nyy = J.Nx;
+    base = J.imgY + slice;
+    row_stride = J.Ny;
+  } else {
+    base = J.imgX + slice;
+    row_stride = J.Nx;
+  }
+  int ylo, yhi;
+  radon_rows((double)X, (double)Y, (double)off - (double)s, 1.0, Nxx, Nyy, ylo, yhi);
+  real acc = 0;
+  for (int y = ylo; y <= yhi; ++y) {
+    const real d = (real)y * Y + off - (real)s;
+    int xl = (int)((-1 - d) * inv);
+    int xh = (int)((1 - d) * inv);
+    if (inv < (real)0) {
+      const int t = xl;
+      xl = xh;
+      xh = t;
+    }
+    xl = xl < 0 ? 0 : xl;
+    xh = xh > Nxx - 1 ? Nxx - 1 : xh;
+    const real *row = base + (int64_t)y * row_stride;
+    for (int x = xl; x <= xh; ++x) {
+      const real w = (real)1 - fabs((real)x * X + d);
+      const real v = row[x];
+      acc = w > (real)0 ? acc + w * v : acc;
+    }
+  }
+  store_fwd(J, s, a, z, acc * (real)delta_x * (real)delta_x / (real)delta_xi);
+}
+
+// Unscaled sum of one fan ray (fanbeam.cl:113-250).
+template <typename real>
+__device__ __forceinline__ real fan_ray_sum(const Job<real> &J, int s, int a, int z) {
+  using A = Arith<real>;
+  const FanStrip<real> F = fan_strip(J, a, z, (real)s - (real)1, (real)s + (real)1);
+  const real smd = (real)s - J.geo[5];
+  int ylo, yhi;
+  fan_rows((double)F.qx, (double)F.qy, (double)F.dpx, (double)F.dmx, (double)F.midx,
+           (double)F.midy, F.Nxx, F.Nyy, ylo, yhi);
+  const real xcap = (real)(F.Nxx - 1) - F.midx;
+  real acc = 0;
+  for (int y = ylo; y <= yhi; ++y) {
+    const real u = (real)y - F.midy;
+    const real xhigh = F.qx + F.dpx * (u - F.qy);
+    const real xlow = F.qx + F.dmx * (u - F.qy);
+    // bounds of the strip in this row; the outer fmax/fmin only keep
+    // infinities (rays parallel to the rows) away from the int conversion
+    const int xh = (int)floor(fmax(fmin(xcap, xhigh) + F.midx, (real)-1));
+    const int xl = (int)ceil(fmin(fmax(-F.midx, xlow) + F.midx, (real)F.Nxx));
+    const real xr = (real)xl - F.midx;
+    real t = F.ddx * xr + F.ddy * u + F.RE;
+    real ss = F.dx * xr + F.dy * u;
+    const real *row = F.base + (int64_t)y * F.row_stride;
+    for (int x = xl; x <= xh; ++x) {
+      const real r = A::rcp(t);
+      const real xi = ss * r;
+      const real w = ((real)1 - fabs(smd - xi)) * r;
+      acc += w * row[x];
+      t += F.ddx;
+      ss += F.dx;
+    }
+  }
+  return acc;
+}
+
+// sino[s] = acc * sdpd[s] / delta_xi * delta_x / R (fanbeam.cl:254-255)
+template <typename real>
+__device__ __forceinline__ void fan_store(const Job<real> &J, int s, int a, int z, real acc) {
+  const real R = J.geo[0], delta_xi = J.geo[2], delta_x = J.geo[10];
+  store_fwd(J, s, a, z, acc * J.sdpd[s] / delta_xi * delta_x / R);
+}
+
+template <typename real>
+__device__ __forceinline__ void fan_fwd_ray(const Job<real> &J, int s, int a, int z) {
+  fan_store(J, s, a, z, fan_ray_sum(J, s, a, z));
+}
+
+// acc: this lane's slot 0 in an LDS array of Slots<K>::kCount rows of 256.
+template <typename real, int K>
+__device__ __forceinline__ void fan_fwd_group(const Job<real> &J, int s0, int a, int z, real *acc) {
+  using A = Arith<real>;
+#pragma unroll
+  for (int k = 0; k < Slots<K>::kCount; ++k) acc[k * 256] = 0;
+  const FanStrip<real> F = fan_strip(J, a, z, (real)(s0 - 1), (real)(s0 + K));
+  if (!F.regular) {
+    for (int k = 0; k < K; ++k) {
+      const int s = s0 + k;
+      if (s < J.Ns) fan_store(J, s, a, z, fan_ray_sum(J, s, a, z));
+    }
+    return;
+  }
+  int ylo, yhi;
+  fan_rows((double)F.qx, (double)F.qy, (double)F.dpx, (double)F.dmx, (double)F.midx,
+           (double)F.midy, F.Nxx, F.Nyy, ylo, yhi);
+  const real xcap = (real)(F.Nxx - 1) - F.midx;
+  // The detector position relative to bin s0 is p = q/t with
+  // q = ss + (mid_det - s0)*t.  q is carried instead of ss (|ss/t| is up to
+  // Ns/2, |q/t| at most K+2), so single precision does not lose the bin
+  // fraction to cancellation; each row starts from a double evaluation.
+  const double c0 = (double)J.geo[5] - (double)s0;
+  const real dq = (real)((double)F.dx + c0 * (double)F.ddx);
+  const real lo = (real)-1.5, hi = (real)K + (real)0.5;
+  for (int y = ylo; y <= yhi; ++y) {
+    const real u = (real)y - F.midy;
+    const real xhigh = F.qx + F.dpx * (u - F.qy);
+    const real xlow = F.qx + F.dmx * (u - F.qy);
+    const int xh = (int)floor(fmax(fmin(xcap, xhigh) + F.midx, (real)-1));
+    const int xl = (int)ceil(fmin(fmax(-F.midx, xlow) + F.midx, (real)F.Nxx));
+    const double xr = (double)xl - (double)F.midx, ud = (double)y - (double)F.midy;
+    const double t0 = (double)F.ddx * xr + (double)F.ddy * ud + (double)F.RE;
+    real t = (real)t0;
+    real q = (real)((double)F.dx * xr + (double)F.dy * ud + c0 * t0);
+    const real *row = F.base + (int64_t)y * F.row_stride;
+    for (int x = xl; x <= xh; ++x) {
+      const real r = A::rcp(t);
+      const real p = A::clamp(q * r, lo, hi);
+      const real fl = floor(p);
+      const int j = (int)fl;
+      const real f = p - fl;
+      const real v = row[x] * r;
+      real *slot = acc + (j + 2) * 256;
+      slot[0] += v - f * v;
+      slot[256] += f * v;
+      t += F.ddx;
+      q += dq;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int s = s0 + k;
+    if (s < J.Ns) fan_store(J, s, a, z, acc[(k + 2) * 256]);
+  }
+}
+
+template <typename real, int K>
+__device__ __forceinline__ void radon_fwd_group(const Job<real> &J, int s0, int a, int z,
+                                                real *acc) {
+  using A = Arith<real>;
+#pragma unroll
+  for (int k = 0; k < Slots<K>::kCount; ++k) acc[k * 256] = 0;
+  const float delta_x = (float)J.geo[0];
+  const float delta_xi = (float)J.geo[1];
+  const real *o = J.ofs + 8 * (int64_t)a;
+  real X = o[0], Y = o[1];
+  const real off = o[2], inv = o[3];
+  int Nxx = J.Nx, Nyy = J.Ny;
+  const real *base;
+  int64_t row_stride;
+  const int64_t slice = (int64_t)z * J.Nx * J.Ny;
+  if (o[5] != (real)0) {
+    const real t = X;
+    X = Y;
+    Y = t;
+    Nxx = J.Ny;
+    Nyy = J.Nx;
+    base = J.imgY + slice;
+    row_stride = J.Ny;
+  } else {
+    base = J.imgX + slice;
+    row_stride = J.Nx;
+  }
+  // union strip of the bins [s0, s0+K): centre s0 + (K-1)/2, half width (K+1)/2
+  int ylo, yhi;
+  radon_rows((double)X, (double)Y, (double)off - ((double)s0 + 0.5 * (K - 1)), 0.5 * (K + 1), Nxx,
+             Nyy, ylo, yhi);
+  const real lo = (real)-1.5, hi = (real)K + (real)0.5;
+  const real sl = (real)(s0 - 1), sh = (real)(s0 + K);
+  for (int y = ylo; y <= yhi; ++y) {
+    const real d = (real)y * Y + off;  // detector position of (0, y)
+    int xl = (int)((sl - d) * inv);
+    int xh = (int)((sh - d) * inv);
+    if (inv < (real)0) {
+      const int t = xl;
+      xl = xh;
+      xh = t;
+    }
+    xl = xl < 0 ? 0 : xl;
+    xh = xh > Nxx - 1 ? Nxx - 1 : xh;
+    // detector position of (xl, y) relative to bin s0, evaluated in double:
+    // x*X and y*Y + off are up to Ns in size and nearly cancel
+    real pr = (real)((double)xl * (double)X + (double)y * (double)Y + (double)off - (double)s0);
+    const real *row = base + (int64_t)y * row_stride;
+    for (int x = xl; x <= xh; ++x) {
+      const real p = A::clamp(pr, lo, hi);
+      const real fl = floor(p);
+      const int j = (int)fl;
+      const real f = p - fl;
+      const real v = row[x];
+      real *slot = acc + (j + 2) * 256;
+      slot[0] += v - f * v;
+      slot[256] += f * v;
+      pr += X;
+    }
+  }
+  const real scale = (real)delta_x * (real)delta_x / (real)delta_xi;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int s = s0 + k;
+    if (s < J.Ns) store_fwd(J, s, a, z, acc[(k + 2) * 256] * scale);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Backprojection prologue and body.
+// ---------------------------------------------------------------------------
+// Per-angle affine table: radon p = c0 x + c1 y + c2; fan t = c0 x + c1 y + c2,
+// ss = c3 x + c4 y + c5 (t = dd.(P - q), ss = d.(P - q), P = (x,y) - mid,
+// dd = (d0 - q)/R^2, d = (xd, yd)/dxi^2; fanbeam.cl:339-355).
+template <typename real>
+__device__ __forceinline__ void back_table_entry(AngleRec<real> *__restrict__ tab,
+                                                 const real *__restrict__ ofs,
+                                                 const real *__restrict__ geo, int fan, int a) {
+  const real *o = ofs + 8 * (int64_t)a;
+  AngleRec<real> rec;
+  for (int k = 0; k < 8; ++k) rec.c[k] = 0;
+  if (!fan) {
+    rec.c[0] = o[0];
+    rec.c[1] = o[1];
+    rec.c[2] = o[2];
+  } else {
+    const double R = geo[0], dxi = geo[2], midx = geo[3], midy = geo[4];
+    const double qx = o[2], qy = o[3];
+    const double ddx = ((double)o[4] - qx) / (R * R), ddy = ((double)o[5] - qy) / (R * R);
+    const double sx = (double)o[0] / (dxi * dxi), sy = (double)o[1] / (dxi * dxi);
+    const double cx = midx + qx, cy = midy + qy;
+    rec.c[0] = (real)ddx;
+    rec.c[1] = (real)ddy;
+    rec.c[2] = (real)(-(ddx * cx + ddy * cy));
+    rec.c[3] = (real)sx;
+    rec.c[4] = (real)sy;
+    rec.c[5] = (real)(-(sx * cx + sy * cy));
+  }
+  tab[a] = rec;
+}
+
+// S[(z*Na + a)*(Ns+4) + j], j = s + 2: sino[s,a,z] * weight[a] (* sdpd[s]/R); zeros at both ends.
+template <typename real>
+__device__ __forceinline__ void stage_entry(real *__restrict__ S, const real *__restrict__ sino,
+                                            Strides3 si, const real *__restrict__ ofs,
+                                            const real *__restrict__ sdpd,
+                                            const real *__restrict__ geo, int fan, int Ns, int Na,
+                                            int j, int a, int z) {
+  const int Nsp = Ns + 4;
+  real v = 0;
+  const int s = j - 2;
+  if (s >= 0 && s < Ns) {
+    const real x = sino[s * si.s0 + a * si.s1 + z * si.s2];
+    if (!fan)
+      v = x * ofs[8 * (int64_t)a + 4];
+    else
+      v = x * sdpd[s] * (ofs[8 * (int64_t)a + 6] / geo[0]);
+  }
+  S[((int64_t)z * Na + a) * Nsp + j] = v;
+}
+
+// RY pixels (x, y0 .. y0+RY-1) of slice z.
+template <typename real, bool FAN, int RY>
+__device__ __forceinline__ void back_pixels(const Job<real> &J, int x, int y0, int z) {
+  using A = Arith<real>;
+  const real xr = (real)x, yr = (real)y0;
+  const int Ns = J.Ns, Na = J.Na;
+  const int Nsp = Ns + 4;
+  const real *Sz = J.S + (int64_t)z * Na * Nsp + 2;
+  const AngleRec<real> *__restrict__ tab = J.tab;
+  const real lo = (real)-1.5, hi = (real)Ns + (real)0.5;
+  const real mid_det = FAN ? J.geo[5] : (real)0;
+  real acc[RY];
+#pragma unroll
+  for (int r = 0; r < RY; ++r) acc[r] = 0;
+#pragma unroll 2
+  for (int a = 0; a < Na; ++a) {
+    const AngleRec<real> &c = tab[a];
+    const real *row = Sz + (int64_t)a * Nsp;
+    real t = FAN ? c.c[0] * xr + c.c[1] * yr + c.c[2] : (real)1;
+    real q = FAN ? c.c[3] * xr + c.c[4] * yr + c.c[5] : c.c[0] * xr + c.c[1] * yr + c.c[2];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      real p, w = 1;
+      if (FAN) {
+        w = A::rcp(t);
+        p = q * w + mid_det;
+      } else {
+        p = q;
+      }
+      p = A::clamp(p, lo, hi);
+      const real fl = floor(p);
+      const int i = (int)fl;
+      const real f = p - fl;
+      const real v0 = row[i], v1 = row[i + 1];
+      const real v = v0 + f * (v1 - v0);
+      acc[r] = FAN ? acc[r] + v * w : acc[r] + v;
+      if (FAN) {
+        t += c.c[1];
+        q += c.c[4];
+      } else {
+        q += c.c[1];
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    const int y = y0 + r;
+    if (y < J.Ny) store_back(J, x, y, z, acc[r]);
+  }
+}
+
+}  // namespace gt

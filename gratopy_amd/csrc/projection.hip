@@ -1,608 +1,85 @@
-// Parallel-beam and fanbeam forward/backprojection for gfx950.
+// Parallel-beam and fanbeam forward/backprojection for gfx950: the
+// single-problem launches of the C ABI.  The device bodies (quadrature,
+// row ranges, bin groups, backprojection gather) live in kernels.h and are
+// shared with the batched launches.
 //
-// Forward (ray-driven gather, one lane per detector bin s of angle a):
-//   sino[s,a,z] = sum over the pixels of the strip between the rays through
-//   detector positions s-1 and s+1 of hat-weight * img, exactly the
-//   quadrature of gratopy/radon.cl:66-161 and gratopy/fanbeam.cl:72-256.
-//   A wave holds 64 consecutive bins of one angle, so each image row it
-//   visits is read as a short contiguous run.  Rays that are closer to the
-//   x axis walk the image by columns; for them the kernel reads a y-fastest
-//   copy of the image made by a tiled transpose, so every wave always reads
-//   along the fastest axis.  Rows that cannot touch the image are skipped
-//   (closed-form row range per ray).
-//
-// Backward (pixel-driven gather, one lane per pixel):
-//   img[x,y,z] = sum_a lerp(S_a, xi_a(x,y)) [ * 1/t_a(x,y) for fan ]
-//   where S is a zero-padded copy of the sinogram already multiplied by the
-//   angle weight (and by sdpd[s]/R for fanbeam), so the inner loop has no
-//   bounds tests and no divergent table loads (gratopy/radon.cl:183-241,
-//   gratopy/fanbeam.cl:290-391).  Per-angle affine coefficients come from a
-//   table built once per call and are read with scalar loads.
+// Forward: a block is 64 lanes (detector bins or bin groups) x 4 angles,
+// grid z = slice.  Rays closer to the x axis read an x-fastest copy of the
+// image, the others a y-fastest copy, so a wave always reads along the
+// fastest axis (gratopy/radon.cl:66-161, gratopy/fanbeam.cl:72-256).
+// Backward: a block is 16 x 16*RY pixels; a prologue builds the per-angle
+// affine table and the zero-padded, pre-weighted sinogram
+// (gratopy/radon.cl:183-241, gratopy/fanbeam.cl:290-391).
 
-#include "common.h"
+#include "kernels.h"
 
 namespace gt {
 
 template <typename real>
-struct alignas(sizeof(real) * 8) AngleRec {
-  real c[8];
-};
-
-// ---------------------------------------------------------------------------
-// Tiled copy between two orders of an N0 x N1 x N2 array (32x32 tiles in LDS).
-// ---------------------------------------------------------------------------
-template <typename real>
 __global__ __launch_bounds__(256) void relayout_kernel(real *__restrict__ dst, Strides3 ds,
                                                        const real *__restrict__ src, Strides3 ss,
-                                                       int N0, int N1, int read0fast,
-                                                       int write0fast) {
+                                                       int N0, int N1) {
   __shared__ real tile[32][33];
-  const int b0 = blockIdx.x * 32, b1 = blockIdx.y * 32;
-  const int64_t z = blockIdx.z;
-  const int lx = threadIdx.x & 31, ly = threadIdx.x >> 5;
-  for (int k = ly; k < 32; k += 8) {
-    const int i0 = read0fast ? b0 + lx : b0 + k;
-    const int i1 = read0fast ? b1 + k : b1 + lx;
-    if (i0 < N0 && i1 < N1) tile[i0 - b0][i1 - b1] = src[i0 * ss.s0 + i1 * ss.s1 + z * ss.s2];
-  }
-  __syncthreads();
-  for (int k = ly; k < 32; k += 8) {
-    const int i0 = write0fast ? b0 + lx : b0 + k;
-    const int i1 = write0fast ? b1 + k : b1 + lx;
-    if (i0 < N0 && i1 < N1) dst[i0 * ds.s0 + i1 * ds.s1 + z * ds.s2] = tile[i0 - b0][i1 - b1];
-  }
+  relayout_tile(tile, dst, ds, src, ss, N0, N1, (int64_t)blockIdx.z, blockIdx.x * 32,
+                blockIdx.y * 32);
 }
 
 template <typename real>
 static void launch_relayout(hipStream_t st, real *dst, Strides3 ds, const real *src, Strides3 ss,
                             int N0, int N1, int N2) {
-  const int read0fast = ss.s0 <= ss.s1;
-  const int write0fast = ds.s0 <= ds.s1;
   dim3 grid((N0 + 31) / 32, (N1 + 31) / 32, N2);
-  hipLaunchKernelGGL(relayout_kernel<real>, grid, dim3(256), 0, st, dst, ds, src, ss, N0, N1,
-                     read0fast, write0fast);
+  hipLaunchKernelGGL(relayout_kernel<real>, grid, dim3(256), 0, st, dst, ds, src, ss, N0, N1);
 }
 
-// ---------------------------------------------------------------------------
-// Row ranges.  Both return a conservative [ylo, yhi]: rows outside it have no
-// pixel inside the ray's strip (one row and one column of margin are added).
-// ---------------------------------------------------------------------------
-__device__ inline void clip_rows(double lo, double hi, int Nyy, int &ylo, int &yhi) {
-  if (lo != lo || hi != hi) {  // NaN: fall back to every row
-    ylo = 0;
-    yhi = Nyy - 1;
-    return;
-  }
-  // clamp in floating point first: infinities must not reach the int conversion
-  lo = fmin(fmax(lo - 1.0, -1.0), (double)Nyy);
-  hi = fmax(fmin(hi + 1.0, (double)Nyy), -1.0);
-  ylo = (int)floor(lo);
-  yhi = (int)ceil(hi);
-  ylo = ylo < 0 ? 0 : ylo;
-  yhi = yhi > Nyy - 1 ? Nyy - 1 : yhi;
-}
-
-// m*u >= L restricts u to a half line (or to nothing / everything).
-__device__ inline void restrict_ge(double m, double L, double &lo, double &hi) {
-  if (m > 0)
-    lo = fmax(lo, L / m);
-  else if (m < 0)
-    hi = fmin(hi, L / m);
-  else if (L > 0) {
-    lo = 1.0;
-    hi = 0.0;
-  }
-}
-
-// Parallel ray (axes already swapped): pixels with |x*X + y*Y + c| < 1 and
-// 0 <= x <= Nxx-1.  The strip centre is g(y) = -(c + y*Y)/X with half width 1/|X|.
-__device__ inline void radon_rows(double X, double Y, double c, int Nxx, int Nyy, int &ylo,
-                                  int &yhi) {
-  const double h = 1.0 / fabs(X);
-  const double g0 = -c / X, m = -Y / X;
-  double lo = -INFINITY, hi = INFINITY;
-  restrict_ge(m, -1.0 - h - g0, lo, hi);           // g(y) >= -1 - h
-  restrict_ge(-m, g0 - ((double)Nxx + h), lo, hi);  // g(y) <= Nxx + h
-  if (lo > hi) {
-    ylo = 0;
-    yhi = -1;
-    return;
-  }
-  clip_rows(lo, hi, Nyy, ylo, yhi);
-}
-
-// Fan ray (axes already swapped): row y is visited on [xlow(y), xhigh(y)] with
-// xhigh = qx + dpx*u, xlow = qx + dmx*u, u = y - midy - qy.
-__device__ inline void fan_rows(double qx, double qy, double dpx, double dmx, double midx,
-                                double midy, int Nxx, int Nyy, int &ylo, int &yhi) {
-  double lo = -INFINITY, hi = INFINITY;
-  restrict_ge(dpx, -midx - 1.0 - qx, lo, hi);              // xhigh >= -midx - 1
-  restrict_ge(-dmx, -((double)Nxx - midx - qx), lo, hi);  // xlow <= Nxx - midx
-  if (lo > hi) {
-    ylo = 0;
-    yhi = -1;
-    return;
-  }
-  clip_rows(lo + midy + qy, hi + midy + qy, Nyy, ylo, yhi);
-}
-
-// ---------------------------------------------------------------------------
-// Forward projections.  Block = 64 bins x 4 angles; grid z = slice.
-// imgX[x + Nx*(y + Ny*z)], imgY[y + Ny*(x + Nx*z)].
-// ---------------------------------------------------------------------------
-template <typename real>
-__global__ __launch_bounds__(256) void radon_fwd_kernel(real *__restrict__ sino, Strides3 so,
-                                                        const real *__restrict__ imgX,
-                                                        const real *__restrict__ imgY,
-                                                        const real *__restrict__ ofs,
-                                                        const real *__restrict__ geo, int Nx,
-                                                        int Ny, int Ns, int Na) {
-  const int s = blockIdx.x * 64 + threadIdx.x;
+// K = 1: one lane per detector bin (the reference's loops); K > 1: a lane
+// owns bins [K*i, K*i + K).
+template <typename real, bool FAN, int K>
+__global__ __launch_bounds__(256) void fwd_kernel(Job<real> J) {
   const int a = blockIdx.y * 4 + threadIdx.y;
   const int z = blockIdx.z;
-  if (s >= Ns || a >= Na) return;
-  // the reference reads the scales as float even in double kernels (radon.cl:82-83)
-  const float delta_x = (float)geo[0];
-  const float delta_xi = (float)geo[1];
-  const real *o = ofs + 8 * (int64_t)a;
-  real X = o[0], Y = o[1];
-  const real off = o[2], inv = o[3];
-  int Nxx = Nx, Nyy = Ny;
-  const real *base;
-  int64_t row_stride;
-  const int64_t slice = (int64_t)z * Nx * Ny;
-  if (o[5] != (real)0) {
-    const real t = X;
-    X = Y;
-    Y = t;
-    Nxx = Ny;
-    Nyy = Nx;
-    base = imgY + slice;
-    row_stride = Ny;
+  if constexpr (K == 1) {
+    const int s = blockIdx.x * 64 + threadIdx.x;
+    if (s >= J.Ns || a >= J.Na) return;
+    if (FAN)
+      fan_fwd_ray(J, s, a, z);
+    else
+      radon_fwd_ray(J, s, a, z);
   } else {
-    base = imgX + slice;
-    row_stride = Nx;
+    __shared__ real slots[Slots<K>::kCount * 256];
+    const int s0 = (blockIdx.x * 64 + threadIdx.x) * K;
+    if (s0 >= J.Ns || a >= J.Na) return;
+    real *acc = slots + threadIdx.y * 64 + threadIdx.x;
+    if (FAN)
+      fan_fwd_group<real, K>(J, s0, a, z, acc);
+    else
+      radon_fwd_group<real, K>(J, s0, a, z, acc);
   }
-  int ylo, yhi;
-  radon_rows((double)X, (double)Y, (double)off - (double)s, Nxx, Nyy, ylo, yhi);
-  real acc = 0;
-  for (int y = ylo; y <= yhi; ++y) {
-    const real d = (real)y * Y + off - (real)s;
-    int xl = (int)((-1 - d) * inv);
-    int xh = (int)((1 - d) * inv);
-    if (inv < (real)0) {
-      const int t = xl;
-      xl = xh;
-      xh = t;
-    }
-    xl = xl < 0 ? 0 : xl;
-    xh = xh > Nxx - 1 ? Nxx - 1 : xh;
-    const real *row = base + (int64_t)y * row_stride;
-    for (int x = xl; x <= xh; ++x) {
-      const real w = (real)1 - fabs((real)x * X + d);
-      const real v = row[x];
-      acc = w > (real)0 ? acc + w * v : acc;
-    }
-  }
-  sino[s * so.s0 + a * so.s1 + z * so.s2] =
-      acc * (real)delta_x * (real)delta_x / (real)delta_xi;
 }
 
 template <typename real>
-__global__ __launch_bounds__(256) void fan_fwd_kernel(real *__restrict__ sino, Strides3 so,
-                                                      const real *__restrict__ imgX,
-                                                      const real *__restrict__ imgY,
-                                                      const real *__restrict__ ofs,
-                                                      const real *__restrict__ sdpd,
-                                                      const real *__restrict__ geo, int Nx,
-                                                      int Ny, int Ns, int Na) {
-  using A = Arith<real>;
-  const int s = blockIdx.x * 64 + threadIdx.x;
-  const int a = blockIdx.y * 4 + threadIdx.y;
-  const int z = blockIdx.z;
-  if (s >= Ns || a >= Na) return;
-  const real R = geo[0];
-  real RE = geo[1];
-  const real delta_xi = geo[2];
-  real midx = geo[3], midy = geo[4];
-  const real mid_det = geo[5];
-  const real delta_x = geo[10];
-  const real *o = ofs + 8 * (int64_t)a;
-  real dx = o[0], dy = o[1], qx = o[2], qy = o[3];
-  const real d0x = o[4], d0y = o[5];
-  // directions from the source to detector positions s+1 and s-1
-  const real sp = -mid_det + (real)s + (real)1;
-  const real sm = -mid_det + (real)s - (real)1;
-  real dpx = d0x + dx * sp - qx, dpy = d0y + dy * sp - qy;
-  real dmx = d0x + dx * sm - qx, dmy = d0y + dy * sm - qy;
-  real ddx = d0x - qx, ddy = d0y - qy;
-  const real RR = R * R;
-  ddx /= RR;
-  ddy /= RR;
-  const real dxi2 = delta_xi * delta_xi;
-  dx /= dxi2;
-  dy /= dxi2;
-  RE *= (real)1 / R;
-  int Nxx = Nx, Nyy = Ny;
-  const real *base;
-  int64_t row_stride;
-  const int64_t slice = (int64_t)z * Nx * Ny;
-  if (fabs(dpx) < fabs(dpy) && fabs(dmx) < fabs(dmy)) {
-    base = imgX + slice;
-    row_stride = Nx;
-  } else {
-    real t;
-    t = ddx; ddx = ddy; ddy = t;
-    t = dx; dx = dy; dy = t;
-    t = qx; qx = qy; qy = t;
-    t = dpx; dpx = dpy; dpy = t;
-    t = dmx; dmx = dmy; dmy = t;
-    t = midx; midx = midy; midy = t;
-    Nyy = Nx;
-    Nxx = Ny;
-    base = imgY + slice;
-    row_stride = Ny;
-  }
-  dpx /= dpy;
-  dmx /= dmy;
-  if (qy * (dmx - dpx) < (real)0) {
-    const real t = dpx;
-    dpx = dmx;
-    dmx = t;
-  }
-  const real smd = (real)s - mid_det;
-  int ylo, yhi;
-  fan_rows((double)qx, (double)qy, (double)dpx, (double)dmx, (double)midx, (double)midy, Nxx,
-           Nyy, ylo, yhi);
-  const real xcap = (real)(Nxx - 1) - midx;
-  real acc = 0;
-  for (int y = ylo; y <= yhi; ++y) {
-    const real u = (real)y - midy;
-    const real xhigh = qx + dpx * (u - qy);
-    const real xlow = qx + dmx * (u - qy);
-    // bounds of the strip in this row; the outer fmax/fmin only keep
-    // infinities (rays parallel to the rows) away from the int conversion
-    const int xh = (int)floor(fmax(fmin(xcap, xhigh) + midx, (real)-1));
-    const int xl = (int)ceil(fmin(fmax(-midx, xlow) + midx, (real)Nxx));
-    const real xr = (real)xl - midx;
-    real t = ddx * xr + ddy * u + RE;
-    real ss = dx * xr + dy * u;
-    const real *row = base + (int64_t)y * row_stride;
-    for (int x = xl; x <= xh; ++x) {
-      const real r = A::rcp(t);
-      const real xi = ss * r;
-      const real w = ((real)1 - fabs(smd - xi)) * r;
-      acc += w * row[x];
-      t += ddx;
-      ss += dx;
-    }
-  }
-  sino[s * so.s0 + a * so.s1 + z * so.s2] = acc * sdpd[s] / delta_xi * delta_x / R;
-}
-
-// ---------------------------------------------------------------------------
-// Forward projections, bin-group form.  A lane owns K adjacent detector bins
-// [s0, s0+K) of one angle and walks, row by row, the union strip between the
-// rays through detector positions s0-1 and s0+K.  Each pixel of the strip is
-// projected once; its two hat weights (1-f, f) go to the bins floor(p) and
-// floor(p)+1 it touches, accumulated in lane-private LDS slots (slot k+2 holds
-// bin s0+k; slots outside [2, K+2) collect the neighbours' share and are
-// dropped).  Per bin, the summation order is the reference's (rows ascending,
-// then along the row), so results agree with the per-ray loops to rounding;
-// the per-row bound computation is shared by K rays and each pixel is
-// projected once instead of twice.
-// ---------------------------------------------------------------------------
-template <int K>
-struct Slots {
-  static constexpr int kCount = K + 4;
-};
-
-template <typename real, int K>
-__global__ __launch_bounds__(256) void fan_fwd_group_kernel(
-    real *__restrict__ sino, Strides3 so, const real *__restrict__ imgX,
-    const real *__restrict__ imgY, const real *__restrict__ ofs, const real *__restrict__ sdpd,
-    const real *__restrict__ geo, int Nx, int Ny, int Ns, int Na) {
-  using A = Arith<real>;
-  __shared__ real slots[Slots<K>::kCount * 256];
-  const int tid = threadIdx.y * 64 + threadIdx.x;
-  const int s0 = (blockIdx.x * 64 + threadIdx.x) * K;
-  const int a = blockIdx.y * 4 + threadIdx.y;
-  const int z = blockIdx.z;
-  if (s0 >= Ns || a >= Na) return;
-  real *acc = slots + tid;
-#pragma unroll
-  for (int k = 0; k < Slots<K>::kCount; ++k) acc[k * 256] = 0;
-
-  const real R = geo[0];
-  real RE = geo[1];
-  const real delta_xi = geo[2];
-  real midx = geo[3], midy = geo[4];
-  const real mid_det = geo[5];
-  const real delta_x = geo[10];
-  const real *o = ofs + 8 * (int64_t)a;
-  real dx = o[0], dy = o[1], qx = o[2], qy = o[3];
-  const real d0x = o[4], d0y = o[5];
-  // rays through detector positions s0 + K (upper) and s0 - 1 (lower)
-  const real sp = -mid_det + (real)(s0 + K);
-  const real sm = -mid_det + (real)(s0 - 1);
-  real dpx = d0x + dx * sp - qx, dpy = d0y + dy * sp - qy;
-  real dmx = d0x + dx * sm - qx, dmy = d0y + dy * sm - qy;
-  real ddx = d0x - qx, ddy = d0y - qy;
-  const real RR = R * R;
-  ddx /= RR;
-  ddy /= RR;
-  const real dxi2 = delta_xi * delta_xi;
-  dx /= dxi2;
-  dy /= dxi2;
-  RE *= (real)1 / R;
-  int Nxx = Nx, Nyy = Ny;
-  const real *base;
-  int64_t row_stride;
-  const int64_t slice = (int64_t)z * Nx * Ny;
-  if (fabs(dpx) < fabs(dpy) && fabs(dmx) < fabs(dmy)) {
-    base = imgX + slice;
-    row_stride = Nx;
-  } else {
-    real t;
-    t = ddx; ddx = ddy; ddy = t;
-    t = dx; dx = dy; dy = t;
-    t = qx; qx = qy; qy = t;
-    t = dpx; dpx = dpy; dpy = t;
-    t = dmx; dmx = dmy; dmy = t;
-    t = midx; midx = midy; midy = t;
-    Nyy = Nx;
-    Nxx = Ny;
-    base = imgY + slice;
-    row_stride = Ny;
-  }
-  dpx /= dpy;
-  dmx /= dmy;
-  if (qy * (dmx - dpx) < (real)0) {
-    const real t = dpx;
-    dpx = dmx;
-    dmx = t;
-  }
-  int ylo, yhi;
-  fan_rows((double)qx, (double)qy, (double)dpx, (double)dmx, (double)midx, (double)midy, Nxx,
-           Nyy, ylo, yhi);
-  const real xcap = (real)(Nxx - 1) - midx;
-  const real c0 = mid_det - (real)s0;  // detector position relative to s0
-  const real lo = (real)-1.5, hi = (real)K + (real)0.5;
-  for (int y = ylo; y <= yhi; ++y) {
-    const real u = (real)y - midy;
-    const real xhigh = qx + dpx * (u - qy);
-    const real xlow = qx + dmx * (u - qy);
-    const int xh = (int)floor(fmax(fmin(xcap, xhigh) + midx, (real)-1));
-    const int xl = (int)ceil(fmin(fmax(-midx, xlow) + midx, (real)Nxx));
-    const real xr = (real)xl - midx;
-    real t = ddx * xr + ddy * u + RE;
-    real ss = dx * xr + dy * u;
-    const real *row = base + (int64_t)y * row_stride;
-    for (int x = xl; x <= xh; ++x) {
-      const real r = A::rcp(t);
-      const real p = A::clamp(ss * r + c0, lo, hi);
-      const real fl = floor(p);
-      const int j = (int)fl;
-      const real f = p - fl;
-      const real v = row[x] * r;
-      real *slot = acc + (j + 2) * 256;
-      slot[0] += v - f * v;
-      slot[256] += f * v;
-      t += ddx;
-      ss += dx;
-    }
-  }
-  const real scale_tail = delta_x / R;
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const int s = s0 + k;
-    if (s < Ns) sino[s * so.s0 + a * so.s1 + z * so.s2] = acc[(k + 2) * 256] * sdpd[s] / delta_xi * scale_tail;
-  }
-}
-
-template <typename real, int K>
-__global__ __launch_bounds__(256) void radon_fwd_group_kernel(
-    real *__restrict__ sino, Strides3 so, const real *__restrict__ imgX,
-    const real *__restrict__ imgY, const real *__restrict__ ofs, const real *__restrict__ geo,
-    int Nx, int Ny, int Ns, int Na) {
-  using A = Arith<real>;
-  __shared__ real slots[Slots<K>::kCount * 256];
-  const int tid = threadIdx.y * 64 + threadIdx.x;
-  const int s0 = (blockIdx.x * 64 + threadIdx.x) * K;
-  const int a = blockIdx.y * 4 + threadIdx.y;
-  const int z = blockIdx.z;
-  if (s0 >= Ns || a >= Na) return;
-  real *acc = slots + tid;
-#pragma unroll
-  for (int k = 0; k < Slots<K>::kCount; ++k) acc[k * 256] = 0;
-  const float delta_x = (float)geo[0];
-  const float delta_xi = (float)geo[1];
-  const real *o = ofs + 8 * (int64_t)a;
-  real X = o[0], Y = o[1];
-  const real off = o[2], inv = o[3];
-  int Nxx = Nx, Nyy = Ny;
-  const real *base;
-  int64_t row_stride;
-  const int64_t slice = (int64_t)z * Nx * Ny;
-  if (o[5] != (real)0) {
-    const real t = X;
-    X = Y;
-    Y = t;
-    Nxx = Ny;
-    Nyy = Nx;
-    base = imgY + slice;
-    row_stride = Ny;
-  } else {
-    base = imgX + slice;
-    row_stride = Nx;
-  }
-  // union strip of the bins [s0, s0+K): centre bin s0 + (K-1)/2, half width (K+1)/2
-  int ylo, yhi;
-  {
-    const double half = 0.5 * (K + 1);
-    const double Xd = X, Yd = Y;
-    const double h = half / fabs(Xd);
-    const double g0 = -((double)off - ((double)s0 + 0.5 * (K - 1))) / Xd, m = -Yd / Xd;
-    double rlo = -INFINITY, rhi = INFINITY;
-    restrict_ge(m, -1.0 - h - g0, rlo, rhi);
-    restrict_ge(-m, g0 - ((double)Nxx + h), rlo, rhi);
-    if (rlo > rhi) {
-      ylo = 0;
-      yhi = -1;
-    } else {
-      clip_rows(rlo, rhi, Nyy, ylo, yhi);
-    }
-  }
-  const real lo = (real)-1.5, hi = (real)K + (real)0.5;
-  const real sl = (real)(s0 - 1), sh = (real)(s0 + K);
-  for (int y = ylo; y <= yhi; ++y) {
-    const real d = (real)y * Y + off;  // detector position of (0, y)
-    int xl = (int)((sl - d) * inv);
-    int xh = (int)((sh - d) * inv);
-    if (inv < (real)0) {
-      const int t = xl;
-      xl = xh;
-      xh = t;
-    }
-    xl = xl < 0 ? 0 : xl;
-    xh = xh > Nxx - 1 ? Nxx - 1 : xh;
-    const real c = d - (real)s0;
-    const real *row = base + (int64_t)y * row_stride;
-    for (int x = xl; x <= xh; ++x) {
-      const real p = A::clamp((real)x * X + c, lo, hi);
-      const real fl = floor(p);
-      const int j = (int)fl;
-      const real f = p - fl;
-      const real v = row[x];
-      real *slot = acc + (j + 2) * 256;
-      slot[0] += v - f * v;
-      slot[256] += f * v;
-    }
-  }
-  const real scale = (real)delta_x * (real)delta_x / (real)delta_xi;
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const int s = s0 + k;
-    if (s < Ns) sino[s * so.s0 + a * so.s1 + z * so.s2] = acc[(k + 2) * 256] * scale;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Backprojection prologue: per-angle affine table and padded weighted sinogram.
-// ---------------------------------------------------------------------------
-template <typename real, bool FAN>
 __global__ void back_table_kernel(AngleRec<real> *__restrict__ tab, const real *__restrict__ ofs,
-                                  const real *__restrict__ geo, int Na) {
+                                  const real *__restrict__ geo, int fan, int Na) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= Na) return;
-  const real *o = ofs + 8 * (int64_t)a;
-  AngleRec<real> rec;
-  for (int k = 0; k < 8; ++k) rec.c[k] = 0;
-  if (!FAN) {
-    rec.c[0] = o[0];
-    rec.c[1] = o[1];
-    rec.c[2] = o[2];
-  } else {
-    // t(x,y) = dd.(P - q) and ss(x,y) = d.(P - q), P = (x,y) - mid, with
-    // dd = (d0 - q)/R^2 and d = (xd, yd)/dxi^2 (gratopy/fanbeam.cl:339-355)
-    const double R = geo[0], dxi = geo[2], midx = geo[3], midy = geo[4];
-    const double qx = o[2], qy = o[3];
-    const double ddx = ((double)o[4] - qx) / (R * R), ddy = ((double)o[5] - qy) / (R * R);
-    const double sx = (double)o[0] / (dxi * dxi), sy = (double)o[1] / (dxi * dxi);
-    const double cx = midx + qx, cy = midy + qy;
-    rec.c[0] = (real)ddx;
-    rec.c[1] = (real)ddy;
-    rec.c[2] = (real)(-(ddx * cx + ddy * cy));
-    rec.c[3] = (real)sx;
-    rec.c[4] = (real)sy;
-    rec.c[5] = (real)(-(sx * cx + sy * cy));
-  }
-  tab[a] = rec;
+  if (a < Na) back_table_entry(tab, ofs, geo, fan, a);
 }
 
-// S[(z*Na + a)*(Ns+4) + 2 + s] = sino[s,a,z] * weight[a] (* sdpd[s]); two zeros on each side.
-template <typename real, bool FAN>
+template <typename real>
 __global__ __launch_bounds__(256) void stage_sino_kernel(real *__restrict__ S,
                                                          const real *__restrict__ sino,
                                                          Strides3 si, const real *__restrict__ ofs,
                                                          const real *__restrict__ sdpd,
-                                                         const real *__restrict__ geo, int Ns,
-                                                         int Na) {
+                                                         const real *__restrict__ geo, int fan,
+                                                         int Ns, int Na) {
   const int j = blockIdx.x * 256 + threadIdx.x;
-  const int a = blockIdx.y;
-  const int z = blockIdx.z;
-  const int Nsp = Ns + 4;
-  if (j >= Nsp) return;
-  real v = 0;
-  const int s = j - 2;
-  if (s >= 0 && s < Ns) {
-    const real x = sino[s * si.s0 + a * si.s1 + z * si.s2];
-    if (!FAN) {
-      v = x * ofs[8 * (int64_t)a + 4];
-    } else {
-      v = x * sdpd[s] * (ofs[8 * (int64_t)a + 6] / geo[0]);
-    }
-  }
-  S[((int64_t)z * Na + a) * Nsp + j] = v;
+  if (j < Ns + 4) stage_entry(S, sino, si, ofs, sdpd, geo, fan, Ns, Na, j, blockIdx.y, blockIdx.z);
 }
 
-// One lane per RY pixels (x, y0..y0+RY-1); a block covers 16 x 16*RY pixels.
-// Per angle the lane evaluates the affine coordinates once and advances them
-// by one row step per pixel; the RY accumulators are independent chains.
 template <typename real, bool FAN, int RY>
-__global__ __launch_bounds__(256) void back_kernel(real *__restrict__ img, Strides3 io,
-                                                   const real *__restrict__ S,
-                                                   const AngleRec<real> *__restrict__ tab,
-                                                   const real *__restrict__ geo, int Nx, int Ny,
-                                                   int Ns, int Na) {
-  using A = Arith<real>;
+__global__ __launch_bounds__(256) void back_kernel(Job<real> J) {
   const int x = blockIdx.x * 16 + (threadIdx.x & 15);
   const int y0 = (blockIdx.y * 16 + (threadIdx.x >> 4)) * RY;
-  const int z = blockIdx.z;
-  if (x >= Nx || y0 >= Ny) return;
-  const real xr = (real)x, yr = (real)y0;
-  const int Nsp = Ns + 4;
-  const real *Sz = S + (int64_t)z * Na * Nsp + 2;
-  const real lo = (real)-1.5, hi = (real)Ns + (real)0.5;
-  const real mid_det = FAN ? geo[5] : (real)0;
-  real acc[RY];
-#pragma unroll
-  for (int r = 0; r < RY; ++r) acc[r] = 0;
-#pragma unroll 2
-  for (int a = 0; a < Na; ++a) {
-    const AngleRec<real> &c = tab[a];
-    const real *row = Sz + (int64_t)a * Nsp;
-    real t = FAN ? c.c[0] * xr + c.c[1] * yr + c.c[2] : (real)1;
-    real q = FAN ? c.c[3] * xr + c.c[4] * yr + c.c[5] : c.c[0] * xr + c.c[1] * yr + c.c[2];
-#pragma unroll
-    for (int r = 0; r < RY; ++r) {
-      real p, w = 1;
-      if (FAN) {
-        w = A::rcp(t);
-        p = q * w + mid_det;
-      } else {
-        p = q;
-      }
-      p = A::clamp(p, lo, hi);
-      const real fl = floor(p);
-      const int i = (int)fl;
-      const real f = p - fl;
-      const real v0 = row[i], v1 = row[i + 1];
-      const real v = v0 + f * (v1 - v0);
-      acc[r] = FAN ? acc[r] + v * w : acc[r] + v;
-      if (FAN) {
-        t += c.c[1];
-        q += c.c[4];
-      } else {
-        q += c.c[1];
-      }
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < RY; ++r) {
-    const int y = y0 + r;
-    if (y < Ny) img[x * io.s0 + y * io.s1 + z * io.s2] = acc[r];
-  }
+  if (x >= J.Nx || y0 >= J.Ny) return;
+  back_pixels<real, FAN, RY>(J, x, y0, blockIdx.z);
 }
 
 // ---------------------------------------------------------------------------
@@ -734,14 +211,30 @@ static size_t elem_size(int dtype) { return dtype == GT_FLOAT64 ? 8 : 4; }
 static size_t fwd_ws(int dtype, const Layout &L, int img_order) {
   const size_t img = align_up(elem_size(dtype) * (size_t)L.Nx * L.Ny * L.Nz);
   size_t n = 0;
-  if (img_order != GT_ORDER_F) n += img;                      // x-fastest copy
-  if (!(img_order == GT_ORDER_C && L.Nz == 1)) n += img;      // y-fastest copy
+  if (img_order != GT_ORDER_F) n += img;                  // x-fastest copy
+  if (!(img_order == GT_ORDER_C && L.Nz == 1)) n += img;  // y-fastest copy
   return n;
 }
 
 static size_t back_ws(int dtype, const Layout &L) {
   return align_up(elem_size(dtype) * 8 * (size_t)L.Na) +
          align_up(elem_size(dtype) * (size_t)(L.Ns + 4) * L.Na * L.Nz);
+}
+
+template <typename real>
+static Job<real> make_job(bool fan, const Layout &L, const real *ofs, const real *sdpd,
+                          const real *geo) {
+  Job<real> J{};
+  J.fan = fan;
+  J.Nx = L.Nx;
+  J.Ny = L.Ny;
+  J.Ns = L.Ns;
+  J.Na = L.Na;
+  J.Nz = L.Nz;
+  J.ofs = ofs;
+  J.sdpd = sdpd;
+  J.geo = geo;
+  return J;
 }
 
 template <typename real, bool FAN>
@@ -756,49 +249,33 @@ static int fwd_impl(hipStream_t st, real *sino, const real *img, const real *ofs
   const Strides3 ys = {L.Ny, 1, (int64_t)L.Nx * L.Ny};  // y fastest
   char *w = static_cast<char *>(ws);
   const size_t img_bytes = align_up(sizeof(real) * (size_t)L.Nx * L.Ny * L.Nz);
-  const real *imgX = img, *imgY = img;
+  Job<real> J = make_job<real>(FAN, L, ofs, sdpd, geo);
+  J.imgX = img;
+  J.imgY = img;
   if (img_order != GT_ORDER_F) {
     real *cp = reinterpret_cast<real *>(w);
     w += img_bytes;
     launch_relayout<real>(st, cp, xs, img, is, L.Nx, L.Ny, L.Nz);
-    imgX = cp;
+    J.imgX = cp;
   }
   if (!(img_order == GT_ORDER_C && L.Nz == 1)) {
     real *cp = reinterpret_cast<real *>(w);
     launch_relayout<real>(st, cp, ys, img, is, L.Nx, L.Ny, L.Nz);
-    imgY = cp;
+    J.imgY = cp;
   }
-  const Strides3 so = order_strides(sino_order, L.Ns, L.Na, L.Nz);
+  J.sino = sino;
+  J.so = order_strides(sino_order, L.Ns, L.Na, L.Nz);
   const int K = tuning().fwd_group;
-  dim3 block(64, 4);
-  dim3 grid((L.Ns + 64 * K - 1) / (64 * K), (L.Na + 3) / 4, L.Nz);
-  if (FAN) {
-    if (K == 8)
-      hipLaunchKernelGGL((fan_fwd_group_kernel<real, 8>), grid, block, 0, st, sino, so, imgX, imgY,
-                         ofs, sdpd, geo, L.Nx, L.Ny, L.Ns, L.Na);
-    else if (K == 4)
-      hipLaunchKernelGGL((fan_fwd_group_kernel<real, 4>), grid, block, 0, st, sino, so, imgX, imgY,
-                         ofs, sdpd, geo, L.Nx, L.Ny, L.Ns, L.Na);
-    else if (K == 2)
-      hipLaunchKernelGGL((fan_fwd_group_kernel<real, 2>), grid, block, 0, st, sino, so, imgX, imgY,
-                         ofs, sdpd, geo, L.Nx, L.Ny, L.Ns, L.Na);
-    else
-      hipLaunchKernelGGL(fan_fwd_kernel<real>, grid, block, 0, st, sino, so, imgX, imgY, ofs, sdpd,
-                         geo, L.Nx, L.Ny, L.Ns, L.Na);
-  } else {
-    if (K == 8)
-      hipLaunchKernelGGL((radon_fwd_group_kernel<real, 8>), grid, block, 0, st, sino, so, imgX,
-                         imgY, ofs, geo, L.Nx, L.Ny, L.Ns, L.Na);
-    else if (K == 4)
-      hipLaunchKernelGGL((radon_fwd_group_kernel<real, 4>), grid, block, 0, st, sino, so, imgX,
-                         imgY, ofs, geo, L.Nx, L.Ny, L.Ns, L.Na);
-    else if (K == 2)
-      hipLaunchKernelGGL((radon_fwd_group_kernel<real, 2>), grid, block, 0, st, sino, so, imgX,
-                         imgY, ofs, geo, L.Nx, L.Ny, L.Ns, L.Na);
-    else
-      hipLaunchKernelGGL(radon_fwd_kernel<real>, grid, block, 0, st, sino, so, imgX, imgY, ofs,
-                         geo, L.Nx, L.Ny, L.Ns, L.Na);
-  }
+  const dim3 block(64, 4);
+  const dim3 grid((L.Ns + 64 * K - 1) / (64 * K), (L.Na + 3) / 4, L.Nz);
+  if (K == 8)
+    hipLaunchKernelGGL((fwd_kernel<real, FAN, 8>), grid, block, 0, st, J);
+  else if (K == 4)
+    hipLaunchKernelGGL((fwd_kernel<real, FAN, 4>), grid, block, 0, st, J);
+  else if (K == 2)
+    hipLaunchKernelGGL((fwd_kernel<real, FAN, 2>), grid, block, 0, st, J);
+  else
+    hipLaunchKernelGGL((fwd_kernel<real, FAN, 1>), grid, block, 0, st, J);
   return check_launch(FAN ? "fan forward projection" : "radon forward projection");
 }
 
@@ -812,23 +289,24 @@ static int back_impl(hipStream_t st, real *img, const real *sino, const real *of
   char *w = static_cast<char *>(ws);
   AngleRec<real> *tab = reinterpret_cast<AngleRec<real> *>(w);
   real *S = reinterpret_cast<real *>(w + align_up(sizeof(real) * 8 * (size_t)L.Na));
-  hipLaunchKernelGGL((back_table_kernel<real, FAN>), dim3((L.Na + 63) / 64), dim3(64), 0, st, tab,
-                     ofs, geo, L.Na);
+  hipLaunchKernelGGL(back_table_kernel<real>, dim3((L.Na + 63) / 64), dim3(64), 0, st, tab, ofs,
+                     geo, (int)FAN, L.Na);
   const Strides3 si = order_strides(sino_order, L.Ns, L.Na, L.Nz);
-  hipLaunchKernelGGL((stage_sino_kernel<real, FAN>), dim3((L.Ns + 4 + 255) / 256, L.Na, L.Nz),
-                     dim3(256), 0, st, S, sino, si, ofs, sdpd, geo, L.Ns, L.Na);
-  const Strides3 io = order_strides(img_order, L.Nx, L.Ny, L.Nz);
+  hipLaunchKernelGGL(stage_sino_kernel<real>, dim3((L.Ns + 4 + 255) / 256, L.Na, L.Nz), dim3(256),
+                     0, st, S, sino, si, ofs, sdpd, geo, (int)FAN, L.Ns, L.Na);
+  Job<real> J = make_job<real>(FAN, L, ofs, sdpd, geo);
+  J.img = img;
+  J.io = order_strides(img_order, L.Nx, L.Ny, L.Nz);
+  J.S = S;
+  J.tab = tab;
   const int RY = tuning().back_rows;
-  dim3 grid((L.Nx + 15) / 16, (L.Ny + 16 * RY - 1) / (16 * RY), L.Nz);
+  const dim3 grid((L.Nx + 15) / 16, (L.Ny + 16 * RY - 1) / (16 * RY), L.Nz);
   if (RY == 4)
-    hipLaunchKernelGGL((back_kernel<real, FAN, 4>), grid, dim3(256), 0, st, img, io, S, tab, geo,
-                       L.Nx, L.Ny, L.Ns, L.Na);
+    hipLaunchKernelGGL((back_kernel<real, FAN, 4>), grid, dim3(256), 0, st, J);
   else if (RY == 2)
-    hipLaunchKernelGGL((back_kernel<real, FAN, 2>), grid, dim3(256), 0, st, img, io, S, tab, geo,
-                       L.Nx, L.Ny, L.Ns, L.Na);
+    hipLaunchKernelGGL((back_kernel<real, FAN, 2>), grid, dim3(256), 0, st, J);
   else
-    hipLaunchKernelGGL((back_kernel<real, FAN, 1>), grid, dim3(256), 0, st, img, io, S, tab, geo,
-                       L.Nx, L.Ny, L.Ns, L.Na);
+    hipLaunchKernelGGL((back_kernel<real, FAN, 1>), grid, dim3(256), 0, st, J);
   return check_launch(FAN ? "fan backprojection" : "radon backprojection");
 }
 

@@ -185,6 +185,29 @@ int gt_tv_primal(gt_stream_t stream, int dtype, void *u_new, void *u_old, const 
                  const void *kstarlambda, float tau, float norming, float z_distance, int Nx,
                  int Ny, int Nz, int order);
 
+/* Vector-Jacobian product of one TV primal-dual step (the step of
+ * gt_tv_dual / gt_tv_update_lambda / gt_tv_primal with K the forward and B the
+ * backprojection).  No reference counterpart: used by the differentiable
+ * unrolled TV reconstruction (gratopy_amd/tv_unrolled.py).  Cotangents carry a
+ * "_next" suffix for the step's outputs.
+ *   gt_tv_adj_primal: wb = [u_next > 0](ub_next + 2 ubarb_next); ub = wb - ubarb_next
+ *   gt_tv_adj_dual:   ab = J(a)(vb_next - tau grad wb), a = v + sigma grad ubar,
+ *                     J the Jacobian of the pointwise unit-ball projection
+ *   gt_tv_adj_div:    ubarb = coef*bterm - sigma div(ab)
+ *   gt_tv_adj_lambda: bb = c(lamb_next - coef_r w_a rwb); lamb = bb; fb -= sigma_p bb;
+ *                     winv = bb / w_a  */
+int gt_tv_adj_primal(gt_stream_t stream, int dtype, void *wb, void *ub, const void *ub_next,
+                     const void *ubarb_next, const void *u_next, size_t n);
+int gt_tv_adj_dual(gt_stream_t stream, int dtype, void *ab, const void *vb_next, const void *wb,
+                   const void *v, const void *ubar, float sigma, float tau, float z_distance,
+                   int Nx, int Ny, int Nz, int order);
+int gt_tv_adj_div(gt_stream_t stream, int dtype, void *ubarb, const void *bterm, double coef,
+                  const void *ab, float sigma, float z_distance, int Nx, int Ny, int Nz,
+                  int order);
+int gt_tv_adj_lambda(gt_stream_t stream, int dtype, void *lamb, void *fb, void *winv,
+                     const void *lamb_next, const void *rwb, const void *angle_weights,
+                     double coef_r, float c, float sigma_p, int Ns, int Na, int Nz, int order);
+
 #ifdef __cplusplus
 }
 #endif

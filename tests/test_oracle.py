@@ -440,6 +440,30 @@ def _div_numpy(v, zd):
 
 
 @pytest.mark.parametrize("order", ["F", "C"])
+@pytest.mark.parametrize("zd", [0.0, 0.6])
+def test_tv_div_is_minus_grad_transpose(order, zd):
+    """<grad u, p> = -<u, div p> for the oracle's update_v / update_u pair
+    (the identity the hand-written TV adjoint relies on)."""
+    rng = np.random.default_rng(21)
+    shape = (7, 6, 5)
+    u = np.require(rng.random(shape), F64, order)
+    p = rng.normal(size=4 * u.size)
+    p[3::4] = 0
+    # grad u via update_v on a zero dual variable with sigma = 1
+    V = np.zeros(4 * u.size)
+    oracle.tv_update_v(V, u, 1.0, zd)
+    # div p via update_u with u_old = 0, tau = 1, no data term (then clamp is inactive: add a shift)
+    shift = 100.0
+    out = np.require(np.zeros(shape), F64, order)
+    oracle.tv_update_u(out, np.require(np.full(shape, shift), F64, order), p,
+                       np.require(np.zeros(shape), F64, order), 1.0, 0.0, zd)
+    div = np.asarray(out) - shift
+    lhs = float(np.dot(V, p))
+    rhs = -float(np.sum(np.asarray(u) * div))
+    assert abs(lhs - rhs) < 1e-12 * max(1.0, abs(lhs))
+
+
+@pytest.mark.parametrize("order", ["F", "C"])
 def test_tv_updates_match_numpy(order):
     """The TV kernels equal an independent numpy gradient/divergence
     (the divergence is minus the adjoint of the gradient)."""
