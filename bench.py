@@ -43,7 +43,151 @@ def parse():
                    help="angles of the bounded CPU-baseline sample (one slice)")
     p.add_argument("--fwd-group", type=int, default=None, help="detector bins per lane (forward)")
     p.add_argument("--back-rows", type=int, default=None, help="pixels per lane (backprojection)")
+    p.add_argument("--no-batched", action="store_true",
+                   help="skip the batched-vs-plain solver measurements")
+    p.add_argument("--tv-iterations", type=int, default=6,
+                   help="TV iterations timed per path on the 2048^2 stack (rate extrapolated to 500)")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="only run an in-place scale of 1 GiB (reads and writes 1 GiB) three times, "
+                        "to calibrate the FETCH_SIZE / WRITE_SIZE counters")
     return p.parse_args()
+
+
+def calibrate_copy():
+    """x *= 0.5 over 2^28 float32 values (1 GiB read + 1 GiB written per launch)."""
+    import torch
+
+    from gratopy_amd import _hip
+
+    _hip.load_library()
+    n = 1 << 28
+    x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    s = torch.cuda.current_stream()
+    for _ in range(3):
+        _hip.call("gt_scale", s.cuda_stream, _hip.dtype_code(x.dtype), x.data_ptr(), 0.5, n)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "gt_scale in place", "bytes_read": 4 * n, "bytes_written": 4 * n,
+                      "launches": 3}), flush=True)
+
+
+def add_noise(sino, percent, rng):
+    """f + (p/100) * (||f||_2 / sqrt(n)) * N(0, 1) on the host."""
+    scale = percent / 100.0 * np.linalg.norm(sino) / np.sqrt(sino.size)
+    return (sino + scale * rng.standard_normal(sino.shape)).astype(sino.dtype)
+
+
+def _timed(fn):
+    import torch
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, out
+
+
+def bench_tv_stack(ps, img, n_iter, nominal=500):
+    """TV (mu=5e-3, 1% noise, slice_thickness 0) on the whole stack: batched
+    (all slices per launch) and graph-captured vs the plain per-slice loop
+    (one slice per call, eager launches).  Both paths run n_iter iterations;
+    slices/s is extrapolated to `nominal` iterations per slice."""
+    import torch
+
+    import gratopy_amd as gratopy
+    from gratopy_amd import _device as dev
+
+    Nz = img.shape[2]
+    sino = gratopy.forwardprojection(img, ps)
+    host = add_noise(dev.to_host(sino), 1.0, np.random.default_rng(7))
+    f = dev.to_device(np.asfortranarray(host), img.device)
+    norm = gratopy.normest(ps)
+    mu = 5e-3
+    slices = [f[:, :, z] for z in range(Nz)]  # Fortran-ordered 2-D views
+    run_batched = lambda: gratopy.total_variation(f, ps, mu, n_iter, slice_thickness=0.0,
+                                                  norm_estimate=norm, graph=True)
+    run_plain = lambda: [gratopy.total_variation(s, ps, mu, n_iter, norm_estimate=norm) for s in slices]
+    run_batched()  # capture / warm-up
+    tb, ub = _timed(run_batched)
+    run_plain()
+    tp, up = _timed(run_plain)
+    diff = max(float((ub[:, :, z] - up[z]).abs().max()) for z in range(Nz))
+    rate_b = Nz / (tb / n_iter * nominal)
+    rate_p = Nz / (tp / n_iter * nominal)
+    return {"metric": "TV(500) slices/s, 32 x 2048^2 fanbeam stack", "batched_graph": round(rate_b, 4),
+            "plain_per_slice": round(rate_p, 4), "speedup": round(rate_b / rate_p, 3),
+            "iterations_timed": n_iter, "ms_per_iteration_batched": round(tb / n_iter * 1e3, 2),
+            "ms_per_iteration_plain": round(tp / n_iter * 1e3, 2),
+            "max_abs_diff_batched_vs_plain": diff}
+
+
+def uneven_mix(n=64, seed=3):
+    """64 slices: Nx from {128,...,1024}, angles from {60,120,360,720},
+    Ns = ceil(1.5 Nx), alternating Radon / fanbeam (RE = Nx, R = 2 Nx,
+    image width Nx, detector width 2 Nx), random-ellipse phantoms,
+    0.5% noise."""
+    import gratopy_amd as gratopy
+    from gratopy_amd.phantom import random_ellipses
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        N = int(rng.choice([128, 256, 384, 512, 768, 1024]))
+        Na = int(rng.choice([60, 120, 360, 720]))
+        Ns = int(np.ceil(1.5 * N))
+        if i % 2 == 0:
+            ps = gratopy.ProjectionSettings("cuda:0", gratopy.RADON, (N, N), Na, Ns)
+        else:
+            ps = gratopy.ProjectionSettings("cuda:0", gratopy.FANBEAM, (N, N), Na, Ns, RE=float(N),
+                                            R=2.0 * N, image_width=float(N), detector_width=2.0 * N)
+        out.append((ps, random_ellipses(N, 10, seed=500 + i, dtype=np.float32)))
+    return out
+
+
+def bench_uneven_landweber(n_iter=100):
+    """Landweber(100) on the 64-slice uneven mix: one batched plan, graph
+    captured, vs the plain per-slice loop of landweber calls."""
+    import torch
+
+    import gratopy_amd as gratopy
+    from gratopy_amd import _device as dev
+    from gratopy_amd.batch import batch_landweber
+
+    mix = uneven_mix()
+    settings = [ps for ps, _ in mix]
+    rng = np.random.default_rng(11)
+    sinos = []
+    for ps, ph in mix:
+        s = dev.to_host(gratopy.forwardprojection(dev.to_device(np.asfortranarray(ph), "cuda:0"), ps))
+        sinos.append(dev.to_device(np.asfortranarray(add_noise(s, 0.5, rng)), "cuda:0"))
+    norms = [gratopy.normest(ps) for ps in settings]
+    work = sum(ps.img_shape[0] * ps.img_shape[1] * ps.sinogram_shape[1] for ps in settings)
+    run_b = lambda: batch_landweber(sinos, settings, n_iter, norm_estimates=norms, graph=True)
+    run_p = lambda: [gratopy.landweber(f, ps, n_iter, norm_estimate=n)
+                     for f, ps, n in zip(sinos, settings, norms)]
+    run_b()
+    tb, ub = _timed(run_b)
+    run_p()
+    tp, up = _timed(run_p)
+    same = all(torch.equal(a, b) for a, b in zip(ub, up))
+    n = len(settings)
+    return {"metric": "Landweber(100) slices/s, 64 uneven slices (config of 128..1024 px, 60..720 angles)",
+            "batched_graph": round(n / tb, 3), "plain_per_slice": round(n / tp, 3),
+            "speedup": round(tp / tb, 3), "fwd+back GUPS batched": round(work * n_iter / tb / 1e9, 2),
+            "bitwise_equal": same}
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 --pmc summary (profiles/pmc_<kernel>_<workload>.txt, line
+    'hbm_bytes_per_launch: N'), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("hbm_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
 
 
 def make_stack(Nz, N, n_ellipses, dtype=np.float32):
@@ -81,6 +225,9 @@ def main():
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (slices shard "
                                    "across GPUs without communication, see DESIGN.md)"}))
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
 
     import torch
 
@@ -132,7 +279,7 @@ def main():
     # dominant launch: the slower of the two directions; algorithmic bytes =
     # 2 interpolation taps x 4 B per pixel-angle per direction
     fwd_med, back_med = float(np.median(t_fwd)), float(np.median(t_back))
-    dom_name, dom_t = ("fan forward", fwd_med) if fwd_med >= back_med else ("fan backprojection", back_med)
+    dom_name, dom_t = ("fwd_kernel", fwd_med) if fwd_med >= back_med else ("back_kernel", back_med)
     algo_bytes = 8.0 * work
     achieved = algo_bytes / dom_t
 
@@ -167,9 +314,18 @@ def main():
         "wall_s": round(wall, 3),
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": None},
+                     "traffic": measured_traffic(args.workload, dom_name)},
         "cpu_baseline": cpu,
     }
+    if not args.no_batched:
+        del sino, back
+        torch.cuda.empty_cache()
+        out["batched"] = {"tv_stack": bench_tv_stack(ps, img, args.tv_iterations)}
+        print("tv stack measured", file=sys.stderr, flush=True)
+        del img
+        torch.cuda.empty_cache()
+        out["batched"]["landweber_uneven"] = bench_uneven_landweber()
+        print("uneven landweber measured", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
 

@@ -73,7 +73,39 @@ PROTOTYPES: dict[str, tuple] = {
     "gt_tv_adj_dual": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _f, _f, _f, _i, _i, _i, _i]),
     "gt_tv_adj_div": (_i, [_vp, _i, _vp, _vp, _d, _vp, _f, _f, _i, _i, _i, _i]),
     "gt_tv_adj_lambda": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _d, _f, _f, _i, _i, _i, _i]),
+    "gt_batch_plan_create": (_i, [ctypes.POINTER(_vp), _i, _i, _vp]),
+    "gt_batch_plan_destroy": (_i, [_vp]),
+    "gt_batch_plan_bytes": (_sz, [_vp]),
+    "gt_batch_launch": (_i, [_vp, _vp, _i, _d]),
 }
+
+# operations of gt_batch_launch
+BATCH_FORWARD = 0
+BATCH_BACKWARD = 1
+BATCH_LANDWEBER = 2
+BATCH_TV_EVEN = 3
+BATCH_TV_ODD = 4
+
+
+class BatchItem(ctypes.Structure):
+    """struct gt_batch_item of include/gratopy_hip.h."""
+
+    _fields_ = [
+        ("geometry", _i),
+        ("Nx", _i),
+        ("Ny", _i),
+        ("Ns", _i),
+        ("Na", _i),
+        ("ofs", _vp),
+        ("sdpd", _vp),
+        ("geo", _vp),
+        ("img", _vp),
+        ("sino", _vp),
+        ("data", _vp),
+        ("img2", _vp),
+        ("field", _vp),
+        ("p", _d * 6),
+    ]
 
 _lib = None
 _lock = threading.Lock()

@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libgratopy_hip.so")
 ARCH = os.environ.get("GRATOPY_HIP_ARCH", "gfx950")
 
-SOURCES = ("runtime.hip", "elementwise.hip", "projection.hip")
+SOURCES = ("runtime.hip", "elementwise.hip", "projection.hip", "batch.hip")
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",

@@ -1,0 +1,408 @@
+// Batched projections over many 2-D slices of uneven geometry (any mix of
+// parallel-beam and fanbeam, image sizes, angle and detector counts) and
+// the Landweber / TV primal-dual iterations over them, a handful of launches
+// per iteration for the whole batch.
+//
+// A plan (gt_batch_plan_create) holds, in one device allocation:
+//   * job records (struct Job of kernels.h), one per slice and table kind
+//     (plain projections, Landweber, TV even / odd iteration);
+//   * the per-angle backprojection tables of every slice (built once);
+//   * per-slice scratch: a y-fastest copy of the image and the padded,
+//     pre-weighted sinogram of the backprojection;
+//   * tile lists: one entry (slice, tile coordinates) per thread block of
+//     each launch kind, sorted by estimated cost (largest first) so the
+//     hardware dispatcher, which hands out blocks in order as slots free,
+//     runs a longest-processing-time-first schedule over the uneven slices.
+// Every launch is a 1-D grid over one tile list; a block reads its tile and
+// its slice's job record with scalar loads and runs the same device bodies
+// as the single-problem kernels, so per slice the results are bitwise those
+// of projection.hip and of the elementwise solver kernels.
+
+#include <algorithm>
+#include <vector>
+
+#include "kernels.h"
+
+struct gt_batch_plan {
+  int dtype;
+  int n_items;
+  int fwd_group, back_rows;
+  bool has_data;  // every item has data (Landweber, TV)
+  bool has_tv;    // every item has data, img2 and field (TV)
+  void *mem;  // one hipMalloc: jobs, tiles, tables, scratch
+  size_t bytes;
+  const void *jobs[4];  // per table kind: Job<real>[n_items]
+  const int4 *tiles[5];
+  int n_tiles[5];
+};
+
+namespace gt {
+
+enum TileKind { kRelayout = 0, kForward = 1, kStage = 2, kBackward = 3, kDual = 4 };
+enum JobKind { kPlain = 0, kLandweber = 1, kTvEven = 2, kTvOdd = 3 };
+
+template <typename real>
+__global__ __launch_bounds__(256) void batch_relayout_kernel(const Job<real> *__restrict__ jobs,
+                                                             const int4 *__restrict__ tiles) {
+  __shared__ real tile[32][33];
+  const int4 t = tiles[blockIdx.x];
+  const Job<real> &J = jobs[t.x];
+  const Strides3 xs = {1, J.Nx, (int64_t)J.Nx * J.Ny};
+  const Strides3 ys = {J.Ny, 1, (int64_t)J.Nx * J.Ny};
+  relayout_tile(tile, const_cast<real *>(J.imgY), ys, J.imgX, xs, J.Nx, J.Ny, 0, t.y, t.z);
+}
+
+template <typename real, int K>
+__global__ __launch_bounds__(256) void batch_fwd_kernel(const Job<real> *__restrict__ jobs,
+                                                        const int4 *__restrict__ tiles) {
+  const int4 t = tiles[blockIdx.x];
+  const Job<real> &J = jobs[t.x];
+  const int a = t.z * 4 + threadIdx.y;
+  if constexpr (K == 1) {
+    const int s = t.y * 64 + threadIdx.x;
+    if (s >= J.Ns || a >= J.Na) return;
+    if (J.fan)
+      fan_fwd_ray(J, s, a, 0);
+    else
+      radon_fwd_ray(J, s, a, 0);
+  } else {
+    __shared__ real slots[Slots<K>::kCount * 256];
+    const int s0 = (t.y * 64 + threadIdx.x) * K;
+    if (s0 >= J.Ns || a >= J.Na) return;
+    real *acc = slots + threadIdx.y * 64 + threadIdx.x;
+    if (J.fan)
+      fan_fwd_group<real, K>(J, s0, a, 0, acc);
+    else
+      radon_fwd_group<real, K>(J, s0, a, 0, acc);
+  }
+}
+
+// tile (slice, angle): one padded sinogram row
+template <typename real>
+__global__ __launch_bounds__(256) void batch_stage_kernel(const Job<real> *__restrict__ jobs,
+                                                          const int4 *__restrict__ tiles) {
+  const int4 t = tiles[blockIdx.x];
+  const Job<real> &J = jobs[t.x];
+  for (int j = threadIdx.x; j < J.Ns + 4; j += 256)
+    stage_entry(const_cast<real *>(J.S), J.sino, J.so, J.ofs, J.sdpd, J.geo, J.fan, J.Ns, J.Na, j,
+                t.y, 0);
+}
+
+template <typename real, int RY>
+__global__ __launch_bounds__(256) void batch_back_kernel(const Job<real> *__restrict__ jobs,
+                                                         const int4 *__restrict__ tiles) {
+  const int4 t = tiles[blockIdx.x];
+  const Job<real> &J = jobs[t.x];
+  const int x = t.y * 16 + (threadIdx.x & 15);
+  const int y0 = (t.z * 16 + (threadIdx.x >> 4)) * RY;
+  if (x >= J.Nx || y0 >= J.Ny) return;
+  if (J.fan)
+    back_pixels<real, true, RY>(J, x, y0, 0);
+  else
+    back_pixels<real, false, RY>(J, x, y0, 0);
+}
+
+// tile (slice, first pixel): 256 pixels of the dual update
+template <typename real>
+__global__ __launch_bounds__(256) void batch_dual_kernel(const Job<real> *__restrict__ jobs,
+                                                         const int4 *__restrict__ tiles,
+                                                         float sigma) {
+  const int4 t = tiles[blockIdx.x];
+  const Job<real> &J = jobs[t.x];
+  const int64_t p = (int64_t)t.y * 256 + threadIdx.x;
+  if (p >= (int64_t)J.Nx * J.Ny) return;
+  tv_dual_pixel(J, const_cast<real *>(J.v), sigma, (int)(p % J.Nx), (int)(p / J.Nx));
+}
+
+template <typename real>
+__global__ void batch_table_kernel(const Job<real> *__restrict__ jobs, int n_items) {
+  const int i = blockIdx.y;
+  if (i >= n_items) return;
+  const Job<real> &J = jobs[i];
+  for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < J.Na; a += gridDim.x * blockDim.x)
+    back_table_entry(const_cast<AngleRec<real> *>(J.tab), J.ofs, J.geo, J.fan, a);
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+// ---------------------------------------------------------------------------
+static int check_item(const gt_batch_item &it) {
+  GT_REQUIRE(it.geometry == GT_GEOMETRY_RADON || it.geometry == GT_GEOMETRY_FAN,
+             "batch item: unknown geometry");
+  GT_REQUIRE(it.Nx > 0 && it.Ny > 0 && it.Ns > 0 && it.Na > 0, "batch item: sizes must be positive");
+  GT_REQUIRE(it.Na <= 65535 && (int64_t)it.Nx * it.Ny < ((int64_t)1 << 31),
+             "batch item: too large");
+  GT_REQUIRE(it.ofs && it.geo, "batch item: NULL table");
+  GT_REQUIRE(it.geometry == GT_GEOMETRY_RADON || it.sdpd, "batch item: NULL sdpd");
+  GT_REQUIRE(it.img && it.sino, "batch item: NULL img or sino");
+  return GT_OK;
+}
+
+struct TileSet {
+  std::vector<int4> t;
+  std::vector<double> cost;
+  void add(int item, int a, int b, double c) {
+    t.push_back(make_int4(item, a, b, 0));
+    cost.push_back(c);
+  }
+  // largest first; ties keep slice order
+  std::vector<int4> sorted() const {
+    std::vector<size_t> idx(t.size());
+    for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+    std::vector<int4> out(t.size());
+    for (size_t i = 0; i < idx.size(); ++i) out[i] = t[idx[i]];
+    return out;
+  }
+};
+
+template <typename real>
+static int plan_create(gt_batch_plan **out, int dtype, int n, const gt_batch_item *items, int K,
+                       int RY) {
+  // ---- layout of the single allocation ----
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off += align_up(bytes);
+    return at;
+  };
+  const size_t jobs_at = take(4 * sizeof(Job<real>) * (size_t)n);
+  std::vector<size_t> tab_at(n), S_at(n), Y_at(n);
+  for (int i = 0; i < n; ++i) {
+    const gt_batch_item &it = items[i];
+    tab_at[i] = take(sizeof(AngleRec<real>) * (size_t)it.Na);
+    S_at[i] = take(sizeof(real) * (size_t)(it.Ns + 4) * it.Na);
+    Y_at[i] = take(sizeof(real) * (size_t)it.Nx * it.Ny);
+  }
+  TileSet sets[5];
+  for (int i = 0; i < n; ++i) {
+    const gt_batch_item &it = items[i];
+    const int nsb = (it.Ns + 64 * K - 1) / (64 * K), nab = (it.Na + 3) / 4;
+    // forward tile: ~ rows crossed x strip width per ray group
+    const double cf = (double)std::max(it.Nx, it.Ny) * (K + 1) * (it.geometry == GT_GEOMETRY_FAN ? 1.5 : 1.0);
+    for (int b = 0; b < nsb; ++b)
+      for (int a = 0; a < nab; ++a) sets[kForward].add(i, b, a, cf);
+    for (int bx = 0; bx < (it.Nx + 31) / 32; ++bx)
+      for (int by = 0; by < (it.Ny + 31) / 32; ++by) sets[kRelayout].add(i, bx * 32, by * 32, 1.0);
+    for (int a = 0; a < it.Na; ++a) sets[kStage].add(i, a, 0, (double)it.Ns);
+    const double cb = (double)it.Na * RY * (it.geometry == GT_GEOMETRY_FAN ? 1.5 : 1.0);
+    for (int bx = 0; bx < (it.Nx + 15) / 16; ++bx)
+      for (int by = 0; by < (it.Ny + 16 * RY - 1) / (16 * RY); ++by) sets[kBackward].add(i, bx, by, cb);
+    const int64_t npix = (int64_t)it.Nx * it.Ny;
+    for (int64_t p = 0; p < (npix + 255) / 256; ++p) sets[kDual].add(i, (int)p, 0, 1.0);
+  }
+  size_t tiles_at[5];
+  std::vector<int4> sorted[5];
+  for (int k = 0; k < 5; ++k) {
+    sorted[k] = sets[k].sorted();
+    tiles_at[k] = take(sizeof(int4) * std::max<size_t>(sorted[k].size(), 1));
+  }
+  void *mem = nullptr;
+  if (hipMalloc(&mem, off) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(GT_EHIP, "batch plan: device allocation failed");
+  }
+  char *base = static_cast<char *>(mem);
+
+  // ---- job records ----
+  std::vector<Job<real>> jobs(4 * (size_t)n);
+  for (int i = 0; i < n; ++i) {
+    const gt_batch_item &it = items[i];
+    Job<real> J{};
+    J.fan = it.geometry == GT_GEOMETRY_FAN;
+    J.Nx = it.Nx;
+    J.Ny = it.Ny;
+    J.Ns = it.Ns;
+    J.Na = it.Na;
+    J.Nz = 1;
+    J.ofs = static_cast<const real *>(it.ofs);
+    J.sdpd = static_cast<const real *>(it.sdpd);
+    J.geo = static_cast<const real *>(it.geo);
+    J.sino = static_cast<real *>(it.sino);
+    J.so = order_strides(GT_ORDER_F, it.Ns, it.Na, 1);
+    J.imgY = reinterpret_cast<const real *>(base + Y_at[i]);
+    J.io = order_strides(GT_ORDER_F, it.Nx, it.Ny, 1);
+    J.S = reinterpret_cast<const real *>(base + S_at[i]);
+    J.tab = reinterpret_cast<const AngleRec<real> *>(base + tab_at[i]);
+    J.data = static_cast<const real *>(it.data);
+    J.v = static_cast<const real *>(it.field);
+    J.sigma_p = (float)it.p[1];
+    J.c = (float)it.p[2];
+    J.tau = (float)it.p[3];
+    J.norming = (float)it.p[4];
+    real *img = static_cast<real *>(it.img);
+    real *img2 = static_cast<real *>(it.img2);
+
+    Job<real> P = J;  // plain: sino = R img, img = B sino
+    P.mode = 0;
+    P.imgX = img;
+    P.img = img;
+    Job<real> L = P;  // Landweber: sino = R img - data; img += p0 * B sino
+    L.mode = 1;
+    L.scale = (real)it.p[0];
+    Job<real> T0 = J;  // TV, even step: u_bar in img2, u in img; u' -> img2, u_bar' -> img
+    T0.mode = 2;
+    T0.imgX = img2;
+    T0.img = img2;
+    T0.u = img;
+    Job<real> T1 = T0;  // TV, odd step: roles swapped
+    T1.imgX = img;
+    T1.img = img;
+    T1.u = img2;
+    jobs[kPlain * (size_t)n + i] = P;
+    jobs[kLandweber * (size_t)n + i] = L;
+    jobs[kTvEven * (size_t)n + i] = T0;
+    jobs[kTvOdd * (size_t)n + i] = T1;
+  }
+  hipError_t e = hipMemcpy(base + jobs_at, jobs.data(), sizeof(Job<real>) * jobs.size(),
+                           hipMemcpyHostToDevice);
+  for (int k = 0; k < 5 && e == hipSuccess; ++k)
+    if (!sorted[k].empty())
+      e = hipMemcpy(base + tiles_at[k], sorted[k].data(), sizeof(int4) * sorted[k].size(),
+                    hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    const Job<real> *dj = reinterpret_cast<const Job<real> *>(base + jobs_at);
+    int maxNa = 1;
+    for (int i = 0; i < n; ++i) maxNa = std::max(maxNa, items[i].Na);
+    hipLaunchKernelGGL(batch_table_kernel<real>, dim3((maxNa + 255) / 256, n), dim3(256), 0, 0, dj,
+                       n);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+  }
+  if (e != hipSuccess) {
+    (void)hipFree(mem);
+    return fail(GT_EHIP, std::string("batch plan: ") + hipGetErrorString(e));
+  }
+  gt_batch_plan *p = new gt_batch_plan{};
+  p->dtype = dtype;
+  p->n_items = n;
+  p->fwd_group = K;
+  p->back_rows = RY;
+  p->has_data = true;
+  p->has_tv = true;
+  for (int i = 0; i < n; ++i) {
+    p->has_data = p->has_data && items[i].data != nullptr;
+    p->has_tv = p->has_tv && items[i].data && items[i].img2 && items[i].field;
+  }
+  p->mem = mem;
+  p->bytes = off;
+  for (int k = 0; k < 4; ++k) p->jobs[k] = base + jobs_at + sizeof(Job<real>) * (size_t)n * k;
+  for (int k = 0; k < 5; ++k) {
+    p->tiles[k] = reinterpret_cast<const int4 *>(base + tiles_at[k]);
+    p->n_tiles[k] = (int)sorted[k].size();
+  }
+  *out = p;
+  return GT_OK;
+}
+
+template <typename real>
+static void launch_fwd(const gt_batch_plan *p, hipStream_t st, const Job<real> *jobs) {
+  if (p->n_tiles[kRelayout])
+    hipLaunchKernelGGL(batch_relayout_kernel<real>, dim3(p->n_tiles[kRelayout]), dim3(256), 0, st,
+                       jobs, p->tiles[kRelayout]);
+  const dim3 grid(p->n_tiles[kForward]), block(64, 4);
+  switch (p->fwd_group) {
+    case 8:
+      hipLaunchKernelGGL((batch_fwd_kernel<real, 8>), grid, block, 0, st, jobs, p->tiles[kForward]);
+      break;
+    case 4:
+      hipLaunchKernelGGL((batch_fwd_kernel<real, 4>), grid, block, 0, st, jobs, p->tiles[kForward]);
+      break;
+    case 2:
+      hipLaunchKernelGGL((batch_fwd_kernel<real, 2>), grid, block, 0, st, jobs, p->tiles[kForward]);
+      break;
+    default:
+      hipLaunchKernelGGL((batch_fwd_kernel<real, 1>), grid, block, 0, st, jobs, p->tiles[kForward]);
+  }
+}
+
+template <typename real>
+static void launch_back(const gt_batch_plan *p, hipStream_t st, const Job<real> *jobs) {
+  hipLaunchKernelGGL(batch_stage_kernel<real>, dim3(p->n_tiles[kStage]), dim3(256), 0, st, jobs,
+                     p->tiles[kStage]);
+  const dim3 grid(p->n_tiles[kBackward]);
+  switch (p->back_rows) {
+    case 4:
+      hipLaunchKernelGGL((batch_back_kernel<real, 4>), grid, dim3(256), 0, st, jobs,
+                         p->tiles[kBackward]);
+      break;
+    case 2:
+      hipLaunchKernelGGL((batch_back_kernel<real, 2>), grid, dim3(256), 0, st, jobs,
+                         p->tiles[kBackward]);
+      break;
+    default:
+      hipLaunchKernelGGL((batch_back_kernel<real, 1>), grid, dim3(256), 0, st, jobs,
+                         p->tiles[kBackward]);
+  }
+}
+
+template <typename real>
+static int launch(const gt_batch_plan *p, hipStream_t st, int op, double sigma) {
+  const Job<real> *J[4];
+  for (int k = 0; k < 4; ++k) J[k] = static_cast<const Job<real> *>(p->jobs[k]);
+  switch (op) {
+    case GT_BATCH_FORWARD:
+      launch_fwd<real>(p, st, J[kPlain]);
+      break;
+    case GT_BATCH_BACKWARD:
+      launch_back<real>(p, st, J[kPlain]);
+      break;
+    case GT_BATCH_LANDWEBER:
+      launch_fwd<real>(p, st, J[kLandweber]);
+      launch_back<real>(p, st, J[kLandweber]);
+      break;
+    case GT_BATCH_TV_EVEN:
+    case GT_BATCH_TV_ODD: {
+      const Job<real> *T = J[op == GT_BATCH_TV_EVEN ? kTvEven : kTvOdd];
+      hipLaunchKernelGGL(batch_dual_kernel<real>, dim3(p->n_tiles[kDual]), dim3(256), 0, st, T,
+                         p->tiles[kDual], (float)sigma);
+      launch_fwd<real>(p, st, T);
+      launch_back<real>(p, st, T);
+      break;
+    }
+    default:
+      return fail(GT_EINVAL, "gt_batch_launch: unknown operation");
+  }
+  return check_launch("batched projection");
+}
+
+}  // namespace gt
+
+using namespace gt;
+
+extern "C" int gt_batch_plan_create(gt_batch_plan **plan, int dtype, int n_items,
+                                    const gt_batch_item *items) {
+  GT_REQUIRE(plan != nullptr, "NULL plan pointer");
+  *plan = nullptr;
+  GT_REQUIRE(dtype == GT_FLOAT32 || dtype == GT_FLOAT64, "dtype must be GT_FLOAT32 or GT_FLOAT64");
+  GT_REQUIRE(n_items > 0 && items != nullptr, "a batch needs at least one item");
+  GT_REQUIRE(n_items <= 65535, "a batch holds at most 65535 items");
+  for (int i = 0; i < n_items; ++i) {
+    const int rc = check_item(items[i]);
+    if (rc) return rc;
+  }
+  const int K = tuning().fwd_group, RY = tuning().back_rows;
+  return dtype == GT_FLOAT64 ? plan_create<double>(plan, dtype, n_items, items, K, RY)
+                             : plan_create<float>(plan, dtype, n_items, items, K, RY);
+}
+
+extern "C" int gt_batch_plan_destroy(gt_batch_plan *plan) {
+  if (!plan) return GT_OK;
+  const hipError_t e = hipFree(plan->mem);
+  delete plan;
+  if (e != hipSuccess) return fail(GT_EHIP, "gt_batch_plan_destroy: hipFree failed");
+  return GT_OK;
+}
+
+extern "C" size_t gt_batch_plan_bytes(const gt_batch_plan *plan) { return plan ? plan->bytes : 0; }
+
+extern "C" int gt_batch_launch(gt_batch_plan *plan, gt_stream_t stream, int op, double sigma) {
+  GT_REQUIRE(plan != nullptr, "NULL plan");
+  GT_REQUIRE(op != GT_BATCH_LANDWEBER || plan->has_data,
+             "gt_batch_launch: Landweber needs data on every item");
+  GT_REQUIRE(!(op == GT_BATCH_TV_EVEN || op == GT_BATCH_TV_ODD) || plan->has_tv,
+             "gt_batch_launch: TV needs data, img2 and field on every item");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return plan->dtype == GT_FLOAT64 ? launch<double>(plan, st, op, sigma)
+                                   : launch<float>(plan, st, op, sigma);
+}

@@ -30,9 +30,17 @@ struct alignas(sizeof(real) * 8) AngleRec {
 };
 
 // Everything one projection needs.  Forward jobs use sino/so/imgX/imgY,
-// backward jobs img/io/S/tab.  Epilogues: forward mode 1 stores
-// value - data[k] (data shares the output's strides); backward mode 1 stores
-// img[k] + scale * value.
+// backward jobs img/io/S/tab.  Epilogues (data shares the sinogram's strides):
+//   mode 0: store the projection;
+//   mode 1: forward stores value - data[k] (Landweber residual), backward
+//           stores img[k] + scale * value (Landweber update);
+//   mode 2: one step of the TV primal-dual iteration of
+//           gratopy/gratopy.py:2554-2762 for a 2-D slice: forward updates
+//           lambda = sino in place, (lambda + sigma_p (value - data)) * c
+//           (total_variation.cl:95-106); backward writes
+//           img = max(0, u + tau (div v - norming value)) and
+//           u <- 2 img - u (total_variation.cl:108-148 and the extragradient
+//           step gratopy.py:2708-2717).
 template <typename real>
 struct Job {
   int fan;
@@ -48,6 +56,9 @@ struct Job {
   int mode;
   const real *data;
   real scale;
+  real *u;
+  const real *v;
+  float sigma_p, c, tau, norming;
 };
 
 template <int K>
@@ -58,18 +69,82 @@ struct Slots {
 // ---------------------------------------------------------------------------
 // epilogues
 // ---------------------------------------------------------------------------
+// The empty asm pins the projection value in a register: the epilogue then
+// starts from the rounded value, as when a separate kernel reads it back,
+// instead of contracting the projection's last multiply into an FMA.
+template <typename real>
+__device__ __forceinline__ void round_barrier(real &v) {
+  asm volatile("" : "+v"(v));
+}
+
 template <typename real>
 __device__ __forceinline__ void store_fwd(const Job<real> &J, int s, int a, int z, real v) {
   const int64_t k = s * J.so.s0 + a * J.so.s1 + z * J.so.s2;
+  if (J.mode != 0) round_barrier(v);
   if (J.mode == 1) v -= J.data[k];
+  if (J.mode == 2) v = (J.sino[k] + (real)J.sigma_p * (v - J.data[k])) * (real)J.c;
   J.sino[k] = v;
+}
+
+// Primal step at pixel (x, y) of a 2-D slice; kstar = (B lambda)(x, y).
+// The divergence is the negative adjoint of the forward-difference gradient
+// with a zero last difference (total_variation.cl:108-148).
+template <typename real>
+__device__ __forceinline__ void tv_primal_store(const Job<real> &J, int x, int y, int64_t i,
+                                                real kstar) {
+  const real *vp = J.v + 4 * i;
+  real d0 = vp[0], d1 = vp[1];
+  if (x == J.Nx - 1) d0 = 0;
+  if (x > 0) d0 -= J.v[4 * (i - J.io.s0) + 0];
+  if (y == J.Ny - 1) d1 = 0;
+  if (y > 0) d1 -= J.v[4 * (i - J.io.s1) + 1];
+  const real d2 = 0;
+  const real old = J.u[i];
+  const real un = fmax((real)0, old + (real)J.tau * (d0 + d1 + d2 - (real)J.norming * kstar));
+  J.img[i] = un;
+  J.u[i] = (real)2 * un - old;
 }
 
 template <typename real>
 __device__ __forceinline__ void store_back(const Job<real> &J, int x, int y, int z, real v) {
   const int64_t k = x * J.io.s0 + y * J.io.s1 + z * J.io.s2;
+  if (J.mode != 0) round_barrier(v);
+  if (J.mode == 2) {
+    tv_primal_store(J, x, y, k, v);
+    return;
+  }
   if (J.mode == 1) v = J.img[k] + J.scale * v;
   J.img[k] = v;
+}
+
+// Dual step at pixel (x, y) of a 2-D slice (total_variation.cl:54-93 and
+// the projection onto the unit ball, 150-167): v += sigma grad(ubar), then
+// v /= max(1, |v|).  ubar is read through imgX (x fastest), v has 4 entries
+// per pixel at 4 * (x + Nx y).
+template <typename real>
+__device__ __forceinline__ void tv_dual_pixel(const Job<real> &J, real *__restrict__ v, float sigma,
+                                              int x, int y) {
+  const int64_t i = x + (int64_t)J.Nx * y;
+  const real *u = J.imgX;
+  const real c = u[i];
+  real g0 = -c, g1 = -c, g2 = -c;
+  g0 = x < J.Nx - 1 ? g0 + u[i + 1] : (real)0;
+  g1 = y < J.Ny - 1 ? g1 + u[i + J.Nx] : (real)0;
+  g2 = (real)0;
+  real *vp = v + 4 * i;
+  const real sg = (real)sigma;
+  real v0 = vp[0] + sg * g0;
+  real v1 = vp[1] + sg * g1;
+  real v2 = vp[2] + sg * g2;
+  const real norm = hypot(hypot(v0, v1), v2);
+  if (norm > (real)1) {
+    v0 /= norm;
+    v1 /= norm;
+    v2 /= norm;
+  }
+  vp[0] = v0;
+  vp[1] = v1;
+  vp[2] = v2;
 }
 
 // ---------------------------------------------------------------------------

@@ -744,27 +744,28 @@ def _iterate(bodies, number_iterations, projectionsetting, graph, label):
     """Run ``bodies[i % len(bodies)](i)`` for i < number_iterations.
 
     Eager mode enqueues every launch from Python, like the reference.  With
-    ``graph=True`` one period (all bodies in order) is captured once into a
-    HIP graph (on the settings' stream, or on torch's capture side stream
-    when the settings use the current stream) and replayed on the stream the
-    eager launches would use; leftover iterations run eagerly.
-    The captured launches are the eager ones, so results are identical.
-    Returns the number of iterations run.
+    ``graph=True`` a block of whole periods (all bodies in order, up to about
+    16 iterations) is captured once into a HIP graph (on the settings'
+    stream, or on torch's capture side stream when the settings use the
+    current stream) and replayed on the stream the eager launches would use;
+    leftover iterations run eagerly.  The captured launches are the eager
+    ones, so results are identical.  Returns the number of iterations run.
     """
     torch = dev.torch
     period = len(bodies)
-    n_graph = (number_iterations // period) if graph else 0
+    per_graph = period * max(1, min(number_iterations // period, 16 // period)) if graph else 0
+    n_graph = (number_iterations // per_graph) if per_graph else 0
     if n_graph:
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g, stream=projectionsetting.queue.stream):
-            for k, body in enumerate(bodies):
-                body(k)
+            for k in range(per_graph):
+                bodies[k % period](k)
         with torch.cuda.stream(_stream_of(projectionsetting)):
             for r in range(n_graph):
                 g.replay()
-                sys.stdout.write("\rProgress at {:3.0%}".format(float(r * period) / number_iterations))
+                sys.stdout.write("\rProgress at {:3.0%}".format(float(r * per_graph) / number_iterations))
         del g
-    for i in range(n_graph * period, number_iterations):
+    for i in range(n_graph * per_graph, number_iterations):
         sys.stdout.write("\rProgress at {:3.0%}".format(float(i) / number_iterations))
         bodies[i % period](i)
     print(f"\r{label} complete")

@@ -3,7 +3,8 @@
  *
  * Every compute entry point is asynchronous on the HIP stream it is given
  * (pass NULL for the null stream), takes raw device pointers and plain sizes,
- * allocates nothing, and returns a status code (GT_OK on success).  The text
+ * allocates nothing (only gt_batch_plan_create allocates, and
+ * gt_batch_plan_destroy frees), and returns a status code (GT_OK on success).  The text
  * of the last failure of the calling thread is available from
  * gt_last_error().  Arrays are dense and either Fortran ordered (GT_ORDER_F:
  * x (or s) fastest, z slowest) or C ordered (GT_ORDER_C: z fastest), exactly
@@ -207,6 +208,60 @@ int gt_tv_adj_div(gt_stream_t stream, int dtype, void *ubarb, const void *bterm,
 int gt_tv_adj_lambda(gt_stream_t stream, int dtype, void *lamb, void *fb, void *winv,
                      const void *lamb_next, const void *rwb, const void *angle_weights,
                      double coef_r, float c, float sigma_p, int Ns, int Na, int Nz, int order);
+
+/* ---- batched slices of uneven geometry -------------------------------------
+ * No single reference entry point: replaces a host loop that calls
+ * forwardprojection / backprojection (gratopy/gratopy.py:126-255), landweber
+ * (gratopy/gratopy.py:2380-2446) or total_variation
+ * (gratopy/gratopy.py:2554-2762) once per slice, each slice with its own
+ * ProjectionSettings.  One item is one 2-D slice; img, img2 (Nx x Ny) and
+ * sino, data (Ns x Na) are Fortran ordered; field holds 4 entries per pixel
+ * (the TV dual variable, laid out as total_variation's V buffer).
+ *
+ * Operations (one gt_batch_launch = one step for every item):
+ *   GT_BATCH_FORWARD   sino = R img
+ *   GT_BATCH_BACKWARD  img = B sino
+ *   GT_BATCH_LANDWEBER sino = R img - data;  img += p[0] * B sino
+ *                      (p[0] = -w / ||R||^2)
+ *   GT_BATCH_TV_EVEN / GT_BATCH_TV_ODD: one Chambolle-Pock iteration of
+ *     total_variation with slice_thickness 0; p = {unused, sigma/||R||,
+ *     mu'/(sigma+mu'), tau, 1/||R||} and `sigma` the dual step.  Even steps
+ *     read u_bar from img2 and u from img and leave the new u in img2 and
+ *     u_bar in img; odd steps swap the roles.  Start from zero img, img2,
+ *     field and sino (lambda); after an even number of steps u is in img.
+ * The plan keeps the items' pointers; results per item are bitwise those of
+ * the single-problem entry points and elementwise TV kernels.  The forward
+ * bin group and backprojection rows per lane are taken from the tuning knobs
+ * (gt_tune_set) when the plan is created.
+ */
+typedef struct gt_batch_item {
+  int geometry; /* GT_GEOMETRY_RADON or GT_GEOMETRY_FAN */
+  int Nx, Ny, Ns, Na;
+  const void *ofs, *sdpd, *geo; /* the item's tables (sdpd NULL for Radon) */
+  void *img, *sino;
+  const void *data; /* measured sinogram (Landweber, TV) */
+  void *img2;       /* second image (TV) */
+  void *field;      /* TV dual variable, 4*Nx*Ny */
+  double p[6];      /* per-item scalars, see above */
+} gt_batch_item;
+
+typedef struct gt_batch_plan gt_batch_plan;
+
+enum {
+  GT_BATCH_FORWARD = 0,
+  GT_BATCH_BACKWARD = 1,
+  GT_BATCH_LANDWEBER = 2,
+  GT_BATCH_TV_EVEN = 3,
+  GT_BATCH_TV_ODD = 4
+};
+
+/* Allocates one device block (job records, tile lists, per-item tables and
+ * scratch) and builds the per-angle tables; synchronizes the device. */
+int gt_batch_plan_create(gt_batch_plan **plan, int dtype, int n_items, const gt_batch_item *items);
+int gt_batch_plan_destroy(gt_batch_plan *plan);
+size_t gt_batch_plan_bytes(const gt_batch_plan *plan);
+/* Asynchronous on `stream`; capturable in a HIP graph (kernel launches only). */
+int gt_batch_launch(gt_batch_plan *plan, gt_stream_t stream, int op, double sigma);
 
 #ifdef __cplusplus
 }

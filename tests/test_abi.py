@@ -78,3 +78,42 @@ def test_workspace_sizes(lib):
     assert lib.gt_projection_workspace_bytes(1, 0, 0, 100, 60, 80, 30, 3, 1) >= 2 * es * 100 * 60 * 3
     # backward: padded sinogram + per-angle table
     assert lib.gt_projection_workspace_bytes(1, 1, 1, 100, 60, 80, 30, 2, 0) >= 8 * (84 * 30 * 2 + 8 * 30)
+
+
+def test_batch_item_layout_matches_the_header(tmp_path):
+    """ctypes' gt_batch_item has the C struct's size and field offsets (gcc on
+    the header)."""
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "gratopy_hip.h"\n'
+        "int main(void){printf(\"%zu %zu %zu %zu %zu\\n\", sizeof(gt_batch_item),"
+        " offsetof(gt_batch_item, ofs), offsetof(gt_batch_item, img),"
+        " offsetof(gt_batch_item, field), offsetof(gt_batch_item, p));return 0;}\n")
+    exe = tmp_path / "layout"
+    import subprocess
+
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True,
+                                          check=True).stdout.split()]
+    B = _hip.BatchItem
+    assert got == [ctypes.sizeof(B), B.ofs.offset, B.img.offset, B.field.offset, B.p.offset]
+
+
+def test_batch_arguments_rejected_without_a_device(lib):
+    plan = ctypes.c_void_p()
+    rc = lib.gt_batch_plan_create(ctypes.byref(plan), 0, 0, None)
+    assert rc == _hip.GT_EINVAL and "at least one" in _hip.last_error()
+    items = (_hip.BatchItem * 2)()
+    for it in items:
+        it.geometry, it.Nx, it.Ny, it.Ns, it.Na = 0, 8, 8, 8, 4
+        it.ofs, it.geo, it.img, it.sino = 1, 2, 3, 4
+    items[1].geometry = 1  # fanbeam without sdpd
+    rc = lib.gt_batch_plan_create(ctypes.byref(plan), 0, 2, ctypes.cast(items, ctypes.c_void_p))
+    assert rc == _hip.GT_EINVAL and "sdpd" in _hip.last_error()
+    items[1].geometry = 5
+    rc = lib.gt_batch_plan_create(ctypes.byref(plan), 0, 2, ctypes.cast(items, ctypes.c_void_p))
+    assert rc == _hip.GT_EINVAL and "geometry" in _hip.last_error()
+    assert plan.value is None
+    assert lib.gt_batch_launch(None, None, 0, 0.0) == _hip.GT_EINVAL
+    assert lib.gt_batch_plan_destroy(None) == _hip.GT_OK
+    assert lib.gt_batch_plan_bytes(None) == 0
