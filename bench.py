@@ -161,6 +161,12 @@ def bench_uneven_landweber(n_iter=100):
         sinos.append(dev.to_device(np.asfortranarray(add_noise(s, 0.5, rng)), "cuda:0"))
     norms = [gratopy.normest(ps) for ps in settings]
     work = sum(ps.img_shape[0] * ps.img_shape[1] * ps.sinogram_shape[1] for ps in settings)
+    from gratopy_amd import autotune
+
+    scratch_i = [dev.empty(ps.img_shape, np.float32, order="F", device="cuda:0") for ps in settings]
+    scratch_s = [dev.empty(ps.sinogram_shape, np.float32, order="F", device="cuda:0") for ps in settings]
+    tuned = autotune.tune_batch(settings, scratch_i, scratch_s)
+    del scratch_i, scratch_s
     run_b = lambda: batch_landweber(sinos, settings, n_iter, norm_estimates=norms, graph=True)
     run_p = lambda: [gratopy.landweber(f, ps, n_iter, norm_estimate=n)
                      for f, ps, n in zip(sinos, settings, norms)]
@@ -173,7 +179,8 @@ def bench_uneven_landweber(n_iter=100):
     return {"metric": "Landweber(100) slices/s, 64 uneven slices (config of 128..1024 px, 60..720 angles)",
             "batched_graph": round(n / tb, 3), "plain_per_slice": round(n / tp, 3),
             "speedup": round(tp / tb, 3), "fwd+back GUPS batched": round(work * n_iter / tb / 1e9, 2),
-            "bitwise_equal": same}
+            "bitwise_equal": same,
+            "tuning": {"fwd_group": tuned["fwd_group"], "back_rows": tuned["back_rows"], "source": "autotuner"}}
 
 
 def measured_traffic(workload, kernel):
@@ -252,6 +259,15 @@ def main():
     sino = dev.empty((Ns, Na, Nz), np.float32, order="F", device=device)
     back = dev.empty((N, N, Nz), np.float32, order="F", device=device)
     stream = torch.cuda.current_stream()
+    tuning_source = "fixed"
+    tuned = None
+    if args.fwd_group is None and args.back_rows is None:
+        from gratopy_amd import autotune
+
+        back.copy_(img)
+        tuned = autotune.tune(ps, back, sino, repeats=2, store=False)
+        tuning_source = "autotuner"
+        print(f"autotuned: {tuned}", file=sys.stderr, flush=True)
 
     for _ in range(args.warmup):
         gratopy.forwardprojection(img, ps, sino=sino)
@@ -308,7 +324,9 @@ def main():
         "config": {"workload": args.workload, "slices": Nz, "image": [N, N], "angles": Na,
                    "detectors": Ns, "RE": RE, "R": R, "image_width": iw, "detector_width": dw,
                    "order": "F"},
-        "tuning": {"fwd_group": lib.gt_tune_get(0), "back_rows": lib.gt_tune_get(1)},
+        "tuning": {"fwd_group": lib.gt_tune_get(0), "back_rows": lib.gt_tune_get(1),
+                   "source": tuning_source,
+                   "trial_ms": None if tuned is None else {"fwd": tuned["fwd_ms"], "back": tuned["back_ms"]}},
         "fwd_ms": round(fwd_med * 1e3, 3),
         "back_ms": round(back_med * 1e3, 3),
         "wall_s": round(wall, 3),

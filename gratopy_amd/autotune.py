@@ -121,6 +121,32 @@ def tune(projectionsetting, img, sino, repeats: int = 3, store: bool = True) -> 
     return best
 
 
+def tune_batch(settings, imgs, sinos, repeats: int = 2) -> dict:
+    """Time all variants on a batch (the tile lists depend on the variant, so
+    one plan is built per variant) and install the fastest.  ``imgs`` and
+    ``sinos`` are overwritten."""
+    from .batch import BatchPlan
+
+    def timed(op):
+        plan = BatchPlan(settings, imgs, sinos)
+        try:
+            return _time(lambda: plan.launch(op), repeats)
+        finally:
+            plan.destroy()
+
+    fwd_times, back_times = {}, {}
+    for k in FWD_GROUPS:
+        set_variant(fwd_group=k)
+        fwd_times[k] = timed(_hip.BATCH_FORWARD)
+    for r in BACK_ROWS:
+        set_variant(back_rows=r)
+        back_times[r] = timed(_hip.BATCH_BACKWARD)
+    best = {"fwd_group": min(fwd_times, key=fwd_times.get), "back_rows": min(back_times, key=back_times.get),
+            "fwd_ms": fwd_times, "back_ms": back_times}
+    set_variant(best["fwd_group"], best["back_rows"])
+    return best
+
+
 def apply(projectionsetting, dtype, nz: int = 1) -> bool:
     """Install the cached variant for this shape; False when none is cached."""
     entry = _load().get(shape_key(projectionsetting, dtype, nz))

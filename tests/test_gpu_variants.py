@@ -1,0 +1,86 @@
+"""Every launch variant the autotuner can pick (forward bin groups K = 1, 2,
+4, 8; backprojection rows per lane RY = 1, 2, 4) meets the parity bar against
+the oracle: normwise 1e-5 in float32, 1e-12 in float64, on regular, shifted,
+ragged and degenerate geometries (including fan bins so wide that a bin
+group's union strip is irregular and the group falls back to per-bin rays)."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import gratopy_amd as gratopy
+from gratopy_amd import autotune
+from tests.helpers import rel_err
+from tests.test_gpu_projection import TOL, _hip, _oracle
+
+pytestmark = pytest.mark.gpu
+
+F32, F64 = np.dtype("float32"), np.dtype("float64")
+
+CASES = [
+    ("radon", (40, 33), 17, 61, {}),
+    ("radon", (5, 29), 3, 1, dict(detector_width=2.5, midpoint_shift=(0.3, -0.2), detector_shift=0.4)),
+    ("fan", (40, 33), 17, 61, dict(R=160.0, RE=70.0, detector_width=120.0)),
+    ("fan", (1, 5), 7, 2, dict(R=800.0, RE=400.0, detector_width=300.0)),
+    ("fan", (5, 29), 3, 1, dict(R=800.0, RE=400.0, detector_width=300.0)),
+    ("fan", (17, 3), 9, 40, dict(R=90.0, RE=40.0, detector_width=60.0, detector_shift=3.0)),
+]
+
+
+@pytest.fixture
+def restore_variant():
+    before = autotune.current_variant()
+    yield
+    autotune.set_variant(before["fwd_group"], before["back_rows"])
+
+
+@pytest.mark.parametrize("K", autotune.FWD_GROUPS)
+def test_forward_variants(K, restore_variant):
+    print(f"variant K={K}", flush=True)
+    autotune.set_variant(fwd_group=K)
+    for geometry, shape, na, nd, kw in CASES:
+        g = gratopy.RADON if geometry == "radon" else gratopy.FANBEAM
+        ps = gratopy.ProjectionSettings("cuda:0", g, shape, na, nd, **kw)
+        img = np.random.default_rng(4).random(shape + (2,))
+        for dtype in (F32, F64):
+            ref = _oracle(ps, img, dtype, False)
+            got = _hip(ps, img, dtype, False)
+            scale = max(np.abs(ref).max(), 1e-3)
+            assert np.abs(got - ref).max() <= TOL[dtype] * scale, (geometry, shape, K, dtype)
+
+
+@pytest.mark.parametrize("RY", autotune.BACK_ROWS)
+def test_backward_variants(RY, restore_variant):
+    print(f"variant RY={RY}", flush=True)
+    autotune.set_variant(back_rows=RY)
+    for geometry, shape, na, nd, kw in CASES:
+        g = gratopy.RADON if geometry == "radon" else gratopy.FANBEAM
+        ps = gratopy.ProjectionSettings("cuda:0", g, shape, na, nd, **kw)
+        sino = np.random.default_rng(5).random(ps.sinogram_shape + (2,))
+        for dtype in (F32, F64):
+            ref = _oracle(ps, sino, dtype, True)
+            got = _hip(ps, sino, dtype, True, in_order="C", out_order="C")
+            scale = max(np.abs(ref).max(), 1e-3)
+            assert np.abs(got - ref).max() <= TOL[dtype] * scale, (geometry, shape, RY, dtype)
+
+
+def test_autotuner_picks_a_variant_and_caches_it(tmp_path, monkeypatch, restore_variant):
+    monkeypatch.setenv("GRATOPY_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    from gratopy_amd import _device as dev
+
+    ps = gratopy.ProjectionSettings("cuda:0", gratopy.FANBEAM, (96, 96), 60, 128, R=400.0, RE=200.0,
+                                    detector_width=250.0)
+    img = dev.to_device(np.asfortranarray(np.random.default_rng(1).random((96, 96)).astype(F32)), "cuda:0")
+    sino = gratopy.forwardprojection(img, ps)
+    best = autotune.tune(ps, img.clone(), sino.clone(), repeats=2)
+    assert best["fwd_group"] in autotune.FWD_GROUPS and best["back_rows"] in autotune.BACK_ROWS
+    assert autotune.current_variant() == {"fwd_group": best["fwd_group"], "back_rows": best["back_rows"]}
+    autotune.set_variant(1, 1)
+    assert autotune.apply(ps, F32, 1)
+    assert autotune.current_variant()["fwd_group"] == best["fwd_group"]
+    # all variants agree to rounding on the tuned geometry
+    ref = dev.to_host(gratopy.forwardprojection(img, ps))
+    for K in autotune.FWD_GROUPS:
+        autotune.set_variant(fwd_group=K)
+        assert rel_err(dev.to_host(gratopy.forwardprojection(img, ps)), ref) < 1e-5
