@@ -51,6 +51,8 @@ struct Arith<float> {
   static __device__ __forceinline__ float clamp(float x, float lo, float hi) {
     return __builtin_amdgcn_fmed3f(x, lo, hi);
   }
+  // x - floor(x), exact (v_fract_f32)
+  static __device__ __forceinline__ float fract(float x) { return __builtin_amdgcn_fractf(x); }
 };
 
 template <>
@@ -59,6 +61,7 @@ struct Arith<double> {
   static __device__ __forceinline__ double clamp(double x, double lo, double hi) {
     return fmin(fmax(x, lo), hi);
   }
+  static __device__ __forceinline__ double fract(double x) { return __builtin_amdgcn_fract(x); }
 };
 
 }  // namespace gt

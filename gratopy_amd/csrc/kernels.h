@@ -660,7 +660,8 @@ __device__ __forceinline__ void radon_fwd_wave(const Job<real> &J, int S0, int a
 // ---------------------------------------------------------------------------
 // Backprojection prologue and body.
 // ---------------------------------------------------------------------------
-// Per-angle affine table: radon p = c0 x + c1 y + c2; fan t = c0 x + c1 y + c2,
+// Per-angle affine table: radon p = c0 x + c1 y + c2 (with c2 = off + 2, the
+// detector coordinate in padded-sinogram indices); fan t = c0 x + c1 y + c2,
 // ss = c3 x + c4 y + c5 (t = dd.(P - q), ss = d.(P - q), P = (x,y) - mid,
 // dd = (d0 - q)/R^2, d = (xd, yd)/dxi^2; fanbeam.cl:339-355).
 template <typename real>
@@ -673,7 +674,7 @@ __device__ __forceinline__ void back_table_entry(AngleRec<real> *__restrict__ ta
   if (!fan) {
     rec.c[0] = o[0];
     rec.c[1] = o[1];
-    rec.c[2] = o[2];
+    rec.c[2] = o[2] + (real)2;
   } else {
     const double R = geo[0], dxi = geo[2], midx = geo[3], midy = geo[4];
     const double qx = o[2], qy = o[3];
@@ -710,17 +711,20 @@ __device__ __forceinline__ void stage_entry(real *__restrict__ S, const real *__
   S[((int64_t)z * Na + a) * Nsp + j] = v;
 }
 
-// RY pixels (x, y0 .. y0+RY-1) of slice z.
+// RY pixels (x, y0 .. y0+RY-1) of slice z.  The detector coordinate p is
+// taken in padded-sinogram indices (bin s at s + 2) and clamped to
+// [0.5, Ns + 2.5], where the two taps floor(p), floor(p) + 1 stay inside the
+// padded row and truncation equals floor.
 template <typename real, bool FAN, int RY>
 __device__ __forceinline__ void back_pixels(const Job<real> &J, int x, int y0, int z) {
   using A = Arith<real>;
   const real xr = (real)x, yr = (real)y0;
   const int Ns = J.Ns, Na = J.Na;
   const int Nsp = Ns + 4;
-  const real *Sz = J.S + (int64_t)z * Na * Nsp + 2;
+  const real *Sz = J.S + (int64_t)z * Na * Nsp;
   const AngleRec<real> *__restrict__ tab = J.tab;
-  const real lo = (real)-1.5, hi = (real)Ns + (real)0.5;
-  const real mid_det = FAN ? J.geo[5] : (real)0;
+  const real lo = (real)0.5, hi = (real)Ns + (real)2.5;
+  const real mid_det = FAN ? J.geo[5] + (real)2 : (real)0;
   real acc[RY];
 #pragma unroll
   for (int r = 0; r < RY; ++r) acc[r] = 0;
@@ -728,22 +732,21 @@ __device__ __forceinline__ void back_pixels(const Job<real> &J, int x, int y0, i
   for (int a = 0; a < Na; ++a) {
     const AngleRec<real> &c = tab[a];
     const real *row = Sz + (int64_t)a * Nsp;
-    real t = FAN ? c.c[0] * xr + c.c[1] * yr + c.c[2] : (real)1;
-    real q = FAN ? c.c[3] * xr + c.c[4] * yr + c.c[5] : c.c[0] * xr + c.c[1] * yr + c.c[2];
+    real t = FAN ? fma(c.c[0], xr, fma(c.c[1], yr, c.c[2])) : (real)1;
+    real q = FAN ? fma(c.c[3], xr, fma(c.c[4], yr, c.c[5])) : fma(c.c[0], xr, fma(c.c[1], yr, c.c[2]));
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
       real p, w = 1;
       if (FAN) {
         w = A::rcp(t);
-        p = q * w + mid_det;
+        p = fma(q, w, mid_det);
       } else {
         p = q;
       }
       p = A::clamp(p, lo, hi);
-      const real fl = floor(p);
-      const int i = (int)fl;
-      const real f = p - fl;
-      const real v0 = row[i], v1 = row[i + 1];
+      const real *tap = row + (int)p;
+      const real f = A::fract(p);
+      const real v0 = tap[0], v1 = tap[1];
       const real v = v0 + f * (v1 - v0);
       acc[r] = FAN ? acc[r] + v * w : acc[r] + v;
       if (FAN) {
