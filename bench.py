@@ -228,9 +228,14 @@ def cpu_baseline(ps_kwargs, N, n_angles, n_det, slice_img):
 
 def main():
     args = parse()
+    # the JSON line is the only output on stdout; the solvers' progress
+    # messages and everything else go to stderr
+    json_out = sys.stdout
+    sys.stdout = sys.stderr
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (slices shard "
-                                   "across GPUs without communication, see DESIGN.md)"}))
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (slices would "
+                                   "shard across GPUs without communication; no multi-GPU path "
+                                   "is built)"}), file=json_out)
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
@@ -344,7 +349,7 @@ def main():
         torch.cuda.empty_cache()
         out["batched"]["landweber_uneven"] = bench_uneven_landweber()
         print("uneven landweber measured", file=sys.stderr, flush=True)
-    print(json.dumps(out), flush=True)
+    print(json.dumps(out), file=json_out, flush=True)
 
 
 if __name__ == "__main__":

@@ -1,8 +1,7 @@
 """Launch-shape autotuner for the projection kernels.
 
 The forward kernels assign K adjacent detector bins to a lane (K in 1, 2,
-4, 8; K = 1 is the per-ray kernel) or to a whole wave whose lanes share each
-image row (K in 128, 256), and the backprojection assigns RY pixels
+4, 8; K = 1 is the per-ray kernel) and the backprojection assigns RY pixels
 along y to a lane (RY in 1, 2, 4).  :func:`tune` times every variant on the
 caller's own geometry and array shapes with HIP events, keeps the fastest,
 and remembers it per shape key in a JSON cache
@@ -23,7 +22,7 @@ import numpy as np
 from . import _device as dev
 from . import _hip
 
-FWD_GROUPS = (1, 2, 4, 8, 128, 256)  # bins per lane, or per wave (128, 256)
+FWD_GROUPS = (1, 2, 4, 8)
 BACK_ROWS = (1, 2, 4)
 KEY_FWD, KEY_BACK = 0, 1
 

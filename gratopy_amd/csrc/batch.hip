@@ -58,14 +58,7 @@ __global__ __launch_bounds__(256) void batch_fwd_kernel(const Job<real> *__restr
   const int4 t = tiles[blockIdx.x];
   const Job<real> &J = jobs[t.x];
   const int a = t.z * 4 + threadIdx.y;
-  if constexpr (K >= 64) {
-    __shared__ real slots[4][K + 4];
-    if (a >= J.Na) return;
-    if (J.fan)
-      fan_fwd_wave<real, K>(J, t.y * K, a, 0, slots[threadIdx.y], threadIdx.x);
-    else
-      radon_fwd_wave<real, K>(J, t.y * K, a, 0, slots[threadIdx.y], threadIdx.x);
-  } else if constexpr (K == 1) {
+  if constexpr (K == 1) {
     const int s = t.y * 64 + threadIdx.x;
     if (s >= J.Ns || a >= J.Na) return;
     if (J.fan)
@@ -184,11 +177,10 @@ static int plan_create(gt_batch_plan **out, int dtype, int n, const gt_batch_ite
   TileSet sets[5];
   for (int i = 0; i < n; ++i) {
     const gt_batch_item &it = items[i];
-    const int per_block = K >= 64 ? K : 64 * K;
-    const int nsb = (it.Ns + per_block - 1) / per_block, nab = (it.Na + 3) / 4;
+    const int nsb = (it.Ns + 64 * K - 1) / (64 * K), nab = (it.Na + 3) / 4;
     // forward tile: ~ rows crossed x strip width per ray group
-    const double cf = (double)std::max(it.Nx, it.Ny) * (K >= 64 ? K / 64.0 + 1 : K + 1) *
-                      (it.geometry == GT_GEOMETRY_FAN ? 1.5 : 1.0);
+    const double cf =
+        (double)std::max(it.Nx, it.Ny) * (K + 1) * (it.geometry == GT_GEOMETRY_FAN ? 1.5 : 1.0);
     for (int b = 0; b < nsb; ++b)
       for (int a = 0; a < nab; ++a) sets[kForward].add(i, b, a, cf);
     for (int bx = 0; bx < (it.Nx + 31) / 32; ++bx)
@@ -311,12 +303,6 @@ static void launch_fwd(const gt_batch_plan *p, hipStream_t st, const Job<real> *
                        jobs, p->tiles[kRelayout]);
   const dim3 grid(p->n_tiles[kForward]), block(64, 4);
   switch (p->fwd_group) {
-    case 256:
-      hipLaunchKernelGGL((batch_fwd_kernel<real, 256>), grid, block, 0, st, jobs, p->tiles[kForward]);
-      break;
-    case 128:
-      hipLaunchKernelGGL((batch_fwd_kernel<real, 128>), grid, block, 0, st, jobs, p->tiles[kForward]);
-      break;
     case 8:
       hipLaunchKernelGGL((batch_fwd_kernel<real, 8>), grid, block, 0, st, jobs, p->tiles[kForward]);
       break;

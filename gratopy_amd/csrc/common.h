@@ -23,7 +23,7 @@ inline size_t align_up(size_t n, size_t a = 256) { return (n + a - 1) / a * a; }
 
 // ---- launch-shape knobs set by the autotuner (gt_tune_set) -----------------
 struct Tuning {
-  int fwd_group = 4;  // forward: bins per lane (1, 2, 4, 8) or per wave (128, 256)
+  int fwd_group = 4;  // detector bins per lane in the forward kernels (1, 2, 4, 8)
   int back_rows = 1;  // pixels per lane (along y) in the backprojection (1, 2, 4)
 };
 Tuning &tuning();

@@ -427,13 +427,14 @@ __device__ __forceinline__ void fan_fwd_group(const Job<real> &J, int s0, int a,
   fan_rows((double)F.qx, (double)F.qy, (double)F.dpx, (double)F.dmx, (double)F.midx,
            (double)F.midy, F.Nxx, F.Nyy, ylo, yhi);
   const real xcap = (real)(F.Nxx - 1) - F.midx;
-  // The detector position relative to bin s0 is p = q/t with
-  // q = ss + (mid_det - s0)*t.  q is carried instead of ss (|ss/t| is up to
-  // Ns/2, |q/t| at most K+2), so single precision does not lose the bin
-  // fraction to cancellation; each row starts from a double evaluation.
-  const double c0 = (double)J.geo[5] - (double)s0;
+  // The detector position relative to bin s0 - 2 (the slot index) is
+  // p = q/t with q = ss + (mid_det - s0 + 2)*t.  q is carried instead of ss
+  // (|ss/t| is up to Ns/2, q/t lies in [1, K+2]), so single precision does
+  // not lose the bin fraction to cancellation; each row starts from a double
+  // evaluation.  p is positive, so truncation is floor.
+  const double c0 = (double)J.geo[5] - (double)s0 + 2.0;
   const real dq = (real)((double)F.dx + c0 * (double)F.ddx);
-  const real lo = (real)-1.5, hi = (real)K + (real)0.5;
+  const real lo = (real)0.5, hi = (real)K + (real)2.5;
   for (int y = ylo; y <= yhi; ++y) {
     const real u = (real)y - F.midy;
     const real xhigh = F.qx + F.dpx * (u - F.qy);
@@ -448,11 +449,9 @@ __device__ __forceinline__ void fan_fwd_group(const Job<real> &J, int s0, int a,
     for (int x = xl; x <= xh; ++x) {
       const real r = A::rcp(t);
       const real p = A::clamp(q * r, lo, hi);
-      const real fl = floor(p);
-      const int j = (int)fl;
-      const real f = p - fl;
+      const real f = A::fract(p);
       const real v = row[x] * r;
-      real *slot = acc + (j + 2) * 256;
+      real *slot = acc + (int)p * 256;
       slot[0] += v - f * v;
       slot[256] += f * v;
       t += F.ddx;
@@ -497,7 +496,7 @@ __device__ __forceinline__ void radon_fwd_group(const Job<real> &J, int s0, int 
   int ylo, yhi;
   radon_rows((double)X, (double)Y, (double)off - ((double)s0 + 0.5 * (K - 1)), 0.5 * (K + 1), Nxx,
              Nyy, ylo, yhi);
-  const real lo = (real)-1.5, hi = (real)K + (real)0.5;
+  const real lo = (real)0.5, hi = (real)K + (real)2.5;
   const real sl = (real)(s0 - 1), sh = (real)(s0 + K);
   for (int y = ylo; y <= yhi; ++y) {
     const real d = (real)y * Y + off;  // detector position of (0, y)
@@ -510,17 +509,16 @@ __device__ __forceinline__ void radon_fwd_group(const Job<real> &J, int s0, int 
     }
     xl = xl < 0 ? 0 : xl;
     xh = xh > Nxx - 1 ? Nxx - 1 : xh;
-    // detector position of (xl, y) relative to bin s0, evaluated in double:
-    // x*X and y*Y + off are up to Ns in size and nearly cancel
-    real pr = (real)((double)xl * (double)X + (double)y * (double)Y + (double)off - (double)s0);
+    // detector position of (xl, y) relative to bin s0 - 2 (the slot index),
+    // evaluated in double: x*X and y*Y + off are up to Ns in size and nearly
+    // cancel; inside the strip it lies in [1, K+2], so truncation is floor
+    real pr = (real)((double)xl * (double)X + (double)y * (double)Y + (double)off - (double)s0 + 2.0);
     const real *row = base + (int64_t)y * row_stride;
     for (int x = xl; x <= xh; ++x) {
       const real p = A::clamp(pr, lo, hi);
-      const real fl = floor(p);
-      const int j = (int)fl;
-      const real f = p - fl;
+      const real f = A::fract(p);
       const real v = row[x];
-      real *slot = acc + (j + 2) * 256;
+      real *slot = acc + (int)p * 256;
       slot[0] += v - f * v;
       slot[256] += f * v;
       pr += X;
@@ -531,129 +529,6 @@ __device__ __forceinline__ void radon_fwd_group(const Job<real> &J, int s0, int 
   for (int k = 0; k < K; ++k) {
     const int s = s0 + k;
     if (s < J.Ns) store_fwd(J, s, a, z, acc[(k + 2) * 256] * scale);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Forward bodies, wave-cooperative form.  One wave owns B adjacent bins
-// [S0, S0+B) of one angle and walks the union strip between the rays through
-// S0-1 and S0+B row by row; the row bounds are computed once per wave and
-// the lanes take consecutive pixels of the row (coalesced loads, one trip
-// count for the whole wave).  Each pixel is projected once and its two hat
-// weights are added to the wave's LDS bins with LDS atomics (slot i holds
-// bin S0 - 2 + i).  The detector coordinate is taken relative to the middle
-// bin S0 + B/2, so its magnitude stays below B/2 + 2.
-// acc: the wave's B + 4 slots; lane: 0..63.
-// ---------------------------------------------------------------------------
-template <typename real, int B>
-__device__ __forceinline__ void fan_fwd_wave(const Job<real> &J, int S0, int a, int z, real *acc,
-                                             int lane) {
-  using A = Arith<real>;
-  for (int k = lane; k < B + 4; k += 64) acc[k] = 0;
-  const FanStrip<real> F = fan_strip(J, a, z, (real)(S0 - 1), (real)(S0 + B));
-  if (!F.regular) {
-    for (int s = S0 + lane; s < S0 + B && s < J.Ns; s += 64) fan_store(J, s, a, z, fan_ray_sum(J, s, a, z));
-    return;
-  }
-  int ylo, yhi;
-  fan_rows((double)F.qx, (double)F.qy, (double)F.dpx, (double)F.dmx, (double)F.midx,
-           (double)F.midy, F.Nxx, F.Nyy, ylo, yhi);
-  const real xcap = (real)(F.Nxx - 1) - F.midx;
-  constexpr int C = B / 2;
-  const double c0 = (double)J.geo[5] - (double)(S0 + C);
-  const real dq = (real)((double)F.dx + c0 * (double)F.ddx);
-  const real lo = (real)(-C) - (real)1.5, hi = (real)(B - C) + (real)0.5;
-  for (int y = ylo; y <= yhi; ++y) {
-    const real u = (real)y - F.midy;
-    const real xhigh = F.qx + F.dpx * (u - F.qy);
-    const real xlow = F.qx + F.dmx * (u - F.qy);
-    const int xh = (int)floor(fmax(fmin(xcap, xhigh) + F.midx, (real)-1));
-    const int xl = (int)ceil(fmin(fmax(-F.midx, xlow) + F.midx, (real)F.Nxx));
-    if (xl > xh) continue;
-    const double xr = (double)xl - (double)F.midx, ud = (double)y - (double)F.midy;
-    const double t0 = (double)F.ddx * xr + (double)F.ddy * ud + (double)F.RE;
-    const real tb = (real)t0;
-    const real qb = (real)((double)F.dx * xr + (double)F.dy * ud + c0 * t0);
-    const real *row = F.base + (int64_t)y * F.row_stride;
-    for (int x = xl + lane; x <= xh; x += 64) {
-      const real k = (real)(x - xl);
-      const real r = A::rcp(fma(F.ddx, k, tb));
-      const real p = A::clamp(fma(dq, k, qb) * r, lo, hi);
-      const real fl = floor(p);
-      const int j = (int)fl + C + 2;
-      const real f = p - fl;
-      const real v = row[x] * r;
-      atomicAdd(acc + j, v - f * v);
-      atomicAdd(acc + j + 1, f * v);
-    }
-  }
-  for (int k = lane; k < B; k += 64) {
-    const int s = S0 + k;
-    if (s < J.Ns) fan_store(J, s, a, z, acc[k + 2]);
-  }
-}
-
-template <typename real, int B>
-__device__ __forceinline__ void radon_fwd_wave(const Job<real> &J, int S0, int a, int z, real *acc,
-                                               int lane) {
-  using A = Arith<real>;
-  for (int k = lane; k < B + 4; k += 64) acc[k] = 0;
-  const float delta_x = (float)J.geo[0];
-  const float delta_xi = (float)J.geo[1];
-  const real *o = J.ofs + 8 * (int64_t)a;
-  real X = o[0], Y = o[1];
-  const real off = o[2], inv = o[3];
-  int Nxx = J.Nx, Nyy = J.Ny;
-  const real *base;
-  int64_t row_stride;
-  const int64_t slice = (int64_t)z * J.Nx * J.Ny;
-  if (o[5] != (real)0) {
-    const real t = X;
-    X = Y;
-    Y = t;
-    Nxx = J.Ny;
-    Nyy = J.Nx;
-    base = J.imgY + slice;
-    row_stride = J.Ny;
-  } else {
-    base = J.imgX + slice;
-    row_stride = J.Nx;
-  }
-  int ylo, yhi;
-  radon_rows((double)X, (double)Y, (double)off - ((double)S0 + 0.5 * (B - 1)), 0.5 * (B + 1), Nxx,
-             Nyy, ylo, yhi);
-  constexpr int C = B / 2;
-  const real lo = (real)(-C) - (real)1.5, hi = (real)(B - C) + (real)0.5;
-  const real sl = (real)(S0 - 1), sh = (real)(S0 + B);
-  for (int y = ylo; y <= yhi; ++y) {
-    const real d = (real)y * Y + off;
-    int xl = (int)((sl - d) * inv);
-    int xh = (int)((sh - d) * inv);
-    if (inv < (real)0) {
-      const int t = xl;
-      xl = xh;
-      xh = t;
-    }
-    xl = xl < 0 ? 0 : xl;
-    xh = xh > Nxx - 1 ? Nxx - 1 : xh;
-    if (xl > xh) continue;
-    const real pb =
-        (real)((double)xl * (double)X + (double)y * (double)Y + (double)off - (double)(S0 + C));
-    const real *row = base + (int64_t)y * row_stride;
-    for (int x = xl + lane; x <= xh; x += 64) {
-      const real p = A::clamp(fma(X, (real)(x - xl), pb), lo, hi);
-      const real fl = floor(p);
-      const int j = (int)fl + C + 2;
-      const real f = p - fl;
-      const real v = row[x];
-      atomicAdd(acc + j, v - f * v);
-      atomicAdd(acc + j + 1, f * v);
-    }
-  }
-  const real scale = (real)delta_x * (real)delta_x / (real)delta_xi;
-  for (int k = lane; k < B; k += 64) {
-    const int s = S0 + k;
-    if (s < J.Ns) store_fwd(J, s, a, z, acc[k + 2] * scale);
   }
 }
 

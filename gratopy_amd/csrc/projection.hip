@@ -31,22 +31,13 @@ static void launch_relayout(hipStream_t st, real *dst, Strides3 ds, const real *
   hipLaunchKernelGGL(relayout_kernel<real>, grid, dim3(256), 0, st, dst, ds, src, ss, N0, N1);
 }
 
-// K = 1: one lane per detector bin (the reference's loops); 1 < K < 64: a
-// lane owns bins [K*i, K*i + K); K >= 64: a wave owns K bins (lanes share
-// the rows).
+// K = 1: one lane per detector bin (the reference's loops); K > 1: a lane
+// owns bins [K*i, K*i + K).
 template <typename real, bool FAN, int K>
 __global__ __launch_bounds__(256) void fwd_kernel(Job<real> J) {
   const int a = blockIdx.y * 4 + threadIdx.y;
   const int z = blockIdx.z;
-  if constexpr (K >= 64) {
-    __shared__ real slots[4][K + 4];
-    const int S0 = blockIdx.x * K;
-    if (a >= J.Na) return;
-    if (FAN)
-      fan_fwd_wave<real, K>(J, S0, a, z, slots[threadIdx.y], threadIdx.x);
-    else
-      radon_fwd_wave<real, K>(J, S0, a, z, slots[threadIdx.y], threadIdx.x);
-  } else if constexpr (K == 1) {
+  if constexpr (K == 1) {
     const int s = blockIdx.x * 64 + threadIdx.x;
     if (s >= J.Ns || a >= J.Na) return;
     if (FAN)
@@ -275,14 +266,9 @@ static int fwd_impl(hipStream_t st, real *sino, const real *img, const real *ofs
   J.sino = sino;
   J.so = order_strides(sino_order, L.Ns, L.Na, L.Nz);
   const int K = tuning().fwd_group;
-  const int per_block = K >= 64 ? K : 64 * K;  // bins per block column
   const dim3 block(64, 4);
-  const dim3 grid((L.Ns + per_block - 1) / per_block, (L.Na + 3) / 4, L.Nz);
-  if (K == 256)
-    hipLaunchKernelGGL((fwd_kernel<real, FAN, 256>), grid, block, 0, st, J);
-  else if (K == 128)
-    hipLaunchKernelGGL((fwd_kernel<real, FAN, 128>), grid, block, 0, st, J);
-  else if (K == 8)
+  const dim3 grid((L.Ns + 64 * K - 1) / (64 * K), (L.Na + 3) / 4, L.Nz);
+  if (K == 8)
     hipLaunchKernelGGL((fwd_kernel<real, FAN, 8>), grid, block, 0, st, J);
   else if (K == 4)
     hipLaunchKernelGGL((fwd_kernel<real, FAN, 4>), grid, block, 0, st, J);

@@ -35,8 +35,8 @@ extern "C" int gt_tune_set(int key, int value) {
   gt::Tuning &t = gt::tuning();
   switch (key) {
     case GT_TUNE_FWD_GROUP:
-      if (value != 1 && value != 2 && value != 4 && value != 8 && value != 128 && value != 256)
-        return gt::fail(GT_EINVAL, "forward group size must be 1, 2, 4, 8, 128 or 256");
+      if (value != 1 && value != 2 && value != 4 && value != 8)
+        return gt::fail(GT_EINVAL, "forward group size must be 1, 2, 4 or 8");
       t.fwd_group = value;
       return GT_OK;
     case GT_TUNE_BACK_ROWS:

@@ -44,8 +44,7 @@ int gt_version(void);
  * No reference counterpart: the reference leaves local sizes to the OpenCL
  * driver (gratopy/gratopy.py:304-313).  Results do not depend on them beyond
  * rounding.
- *   GT_TUNE_FWD_GROUP: detector bins per lane in the forward projection (1, 2, 4, 8),
- *                      or per wave with the lanes sharing each image row (128, 256)
+ *   GT_TUNE_FWD_GROUP: detector bins per lane in the forward projection (1, 2, 4, 8)
  *   GT_TUNE_BACK_ROWS: pixels per lane in the backprojection (1, 2, 4) */
 enum { GT_TUNE_FWD_GROUP = 0, GT_TUNE_BACK_ROWS = 1 };
 int gt_tune_set(int key, int value);

@@ -102,8 +102,28 @@ def test_cg_matches_oracle(geometry, dtype, order):
 
     if dtype == F64:
         assert rel_err(run(sino, False), run(sino, True)) < 1e-8
-    else:
-        _vs_float64(run, sino, f"cg {geometry} {order}")
+        return
+    # Float32 CG on the normal equations of this under-determined problem
+    # amplifies any change of rounding: perturbing the oracle's projections by
+    # a relative 1e-7 (random, per element) moves the 8-step float32 solution
+    # by ~6e-3, 1e-6 by ~1.5e-2, so solutions of two float32 implementations
+    # are not comparable element-wise.  What CG minimises is: the relative
+    # data residual ||R x - f|| / ||f|| of the HIP float32 solution must be
+    # within 10% of that of the float64 solution (float64 oracle operator).
+    s64 = np.require(sino.astype(np.float64), None, order)
+    truth = run(s64, True)
+    got = run(sino, False)
+    ref32 = run(sino, True)
+
+    def residual(u):
+        r = g.fwd(np.require(np.asarray(u, np.float64), None, order), order) - s64
+        return np.linalg.norm(r) / np.linalg.norm(s64)
+
+    r_hip, r_ref, r_64 = residual(got), residual(ref32), residual(truth)
+    print(f"cg {geometry} {order} float32: residual hip {r_hip:.5e}, reference algorithm {r_ref:.5e}, "
+          f"float64 {r_64:.5e}; solution error vs float64: hip {rel_err(got, truth):.3e}, "
+          f"reference algorithm {rel_err(ref32, truth):.3e}", flush=True)
+    assert r_hip <= 1.1 * r_64
 
 
 def test_cg_early_stop():
@@ -203,11 +223,23 @@ def test_walnut_landweber():
     assert_control(u, (600, 600, 328, 120, 2), 0.971266, "img", "Landweber reconstruction")
 
 
+# The reference's CG control number 0.75674 (tests/test_fanbeam.py:1167-1239)
+# is that of its float32 run: the oracle in float32 reproduces it (0.756738),
+# exact CG (the oracle in float64) gives 0.759492 on the same data
+# (tests/test_oracle_walnut.py).  Fifty float32 CG steps on this
+# under-determined problem follow their own rounding; the reference accepts
+# rtol 1e-2 on CPU OpenCL platforms for that reason (test_fanbeam.py:1232-1237).
+# The HIP float32 run must meet that, and the float64 value to rtol 1e-3.
+WALNUT_CG_FLOAT64 = 0.7594917441
+
+
 def test_walnut_conjugate_gradients():
     print("walnut cg", flush=True)
     ps = _walnut_ps((600, 600))
     u = dev.to_host(gratopy.conjugate_gradients(_walnut_two_slices(ps), ps, number_iterations=50))
-    assert_control(u, (600, 600, 328, 120, 2), 0.75674, "img", "CG reconstruction")
+    dims = (600, 600, 328, 120, 2)
+    assert_control(u, dims, 0.75674, "img", "CG reconstruction", rtol=1e-2)
+    assert_control(u, dims, WALNUT_CG_FLOAT64, "img", "CG reconstruction (float64 value)")
 
 
 def test_walnut_total_variation():
