@@ -409,9 +409,33 @@ __device__ __forceinline__ void fan_fwd_ray(const Job<real> &J, int s, int a, in
   fan_store(J, s, a, z, fan_ray_sum(J, s, a, z));
 }
 
+// Row band of a banded forward launch: the strip walk is restricted to rows
+// [index*Nyy/count, (index+1)*Nyy/count) of its frame and the raw per-bin
+// sums go to part[((index*Nz + z)*Na + a)*Ns + s]; a second pass adds the
+// bands in order and applies the scale and epilogue (band_reduce_entry).
+template <typename real>
+struct Band {
+  int index, count;
+  real *part;
+};
+
+template <typename real>
+__device__ __forceinline__ real *band_slot(const Job<real> &J, const Band<real> &B, int s, int a,
+                                           int z) {
+  return B.part + (((int64_t)B.index * J.Nz + z) * J.Na + a) * J.Ns + s;
+}
+
+__device__ __forceinline__ void band_rows(int Nyy, int index, int count, int &ylo, int &yhi) {
+  const int r0 = (int)((int64_t)Nyy * index / count);
+  const int r1 = (int)((int64_t)Nyy * (index + 1) / count) - 1;
+  ylo = ylo > r0 ? ylo : r0;
+  yhi = yhi < r1 ? yhi : r1;
+}
+
 // acc: this lane's slot 0 in an LDS array of Slots<K>::kCount rows of 256.
-template <typename real, int K>
-__device__ __forceinline__ void fan_fwd_group(const Job<real> &J, int s0, int a, int z, real *acc) {
+template <typename real, int K, bool BANDED = false>
+__device__ __forceinline__ void fan_fwd_group(const Job<real> &J, int s0, int a, int z, real *acc,
+                                              Band<real> B = {}) {
   using A = Arith<real>;
 #pragma unroll
   for (int k = 0; k < Slots<K>::kCount; ++k) acc[k * 256] = 0;
@@ -419,13 +443,18 @@ __device__ __forceinline__ void fan_fwd_group(const Job<real> &J, int s0, int a,
   if (!F.regular) {
     for (int k = 0; k < K; ++k) {
       const int s = s0 + k;
-      if (s < J.Ns) fan_store(J, s, a, z, fan_ray_sum(J, s, a, z));
+      if (s >= J.Ns) break;
+      if (!BANDED)
+        fan_store(J, s, a, z, fan_ray_sum(J, s, a, z));
+      else  // the whole ray goes to band 0
+        *band_slot(J, B, s, a, z) = B.index == 0 ? fan_ray_sum(J, s, a, z) : (real)0;
     }
     return;
   }
   int ylo, yhi;
   fan_rows((double)F.qx, (double)F.qy, (double)F.dpx, (double)F.dmx, (double)F.midx,
            (double)F.midy, F.Nxx, F.Nyy, ylo, yhi);
+  if (BANDED) band_rows(F.Nyy, B.index, B.count, ylo, yhi);
   const real xcap = (real)(F.Nxx - 1) - F.midx;
   // The detector position relative to bin s0 - 2 (the slot index) is
   // p = q/t with q = ss + (mid_det - s0 + 2)*t.  q is carried instead of ss
@@ -461,18 +490,28 @@ __device__ __forceinline__ void fan_fwd_group(const Job<real> &J, int s0, int a,
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int s = s0 + k;
-    if (s < J.Ns) fan_store(J, s, a, z, acc[(k + 2) * 256]);
+    if (s >= J.Ns) continue;
+    if (BANDED)
+      *band_slot(J, B, s, a, z) = acc[(k + 2) * 256];
+    else
+      fan_store(J, s, a, z, acc[(k + 2) * 256]);
   }
 }
 
-template <typename real, int K>
+// float(dx)^2 / float(ds), the reference's Radon scale (radon.cl:82-83, 159-160)
+template <typename real>
+__device__ __forceinline__ real radon_scale(const Job<real> &J) {
+  const float delta_x = (float)J.geo[0];
+  const float delta_xi = (float)J.geo[1];
+  return (real)delta_x * (real)delta_x / (real)delta_xi;
+}
+
+template <typename real, int K, bool BANDED = false>
 __device__ __forceinline__ void radon_fwd_group(const Job<real> &J, int s0, int a, int z,
-                                                real *acc) {
+                                                real *acc, Band<real> B = {}) {
   using A = Arith<real>;
 #pragma unroll
   for (int k = 0; k < Slots<K>::kCount; ++k) acc[k * 256] = 0;
-  const float delta_x = (float)J.geo[0];
-  const float delta_xi = (float)J.geo[1];
   const real *o = J.ofs + 8 * (int64_t)a;
   real X = o[0], Y = o[1];
   const real off = o[2], inv = o[3];
@@ -496,6 +535,7 @@ __device__ __forceinline__ void radon_fwd_group(const Job<real> &J, int s0, int 
   int ylo, yhi;
   radon_rows((double)X, (double)Y, (double)off - ((double)s0 + 0.5 * (K - 1)), 0.5 * (K + 1), Nxx,
              Nyy, ylo, yhi);
+  if (BANDED) band_rows(Nyy, B.index, B.count, ylo, yhi);
   const real lo = (real)0.5, hi = (real)K + (real)2.5;
   const real sl = (real)(s0 - 1), sh = (real)(s0 + K);
   for (int y = ylo; y <= yhi; ++y) {
@@ -524,12 +564,31 @@ __device__ __forceinline__ void radon_fwd_group(const Job<real> &J, int s0, int 
       pr += X;
     }
   }
-  const real scale = (real)delta_x * (real)delta_x / (real)delta_xi;
+  const real scale = radon_scale(J);
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int s = s0 + k;
-    if (s < J.Ns) store_fwd(J, s, a, z, acc[(k + 2) * 256] * scale);
+    if (s >= J.Ns) continue;
+    if (BANDED)
+      *band_slot(J, B, s, a, z) = acc[(k + 2) * 256];
+    else
+      store_fwd(J, s, a, z, acc[(k + 2) * 256] * scale);
   }
+}
+
+// Sum of the row bands of bin (s, a, z) in band order, then the per-ray scale
+// and epilogue of the unbanded bodies.
+template <typename real>
+__device__ __forceinline__ void band_reduce_entry(const Job<real> &J, const real *__restrict__ part,
+                                                  int count, int s, int a, int z) {
+  const int64_t stride = (int64_t)J.Nz * J.Na * J.Ns;
+  const real *p = part + ((int64_t)z * J.Na + a) * J.Ns + s;
+  real acc = p[0];
+  for (int b = 1; b < count; ++b) acc += p[b * stride];
+  if (J.fan)
+    fan_store(J, s, a, z, acc);
+  else
+    store_fwd(J, s, a, z, acc * radon_scale(J));
 }
 
 // ---------------------------------------------------------------------------

@@ -56,6 +56,32 @@ __global__ __launch_bounds__(256) void fwd_kernel(Job<real> J) {
   }
 }
 
+// Banded form of the bin-group forward: blockIdx.x = band + nbands * group,
+// so consecutive blocks, which the dispatcher hands to the 8 XCDs in turn,
+// walk different row bands and each XCD's L2 serves one band of the image.
+template <typename real, bool FAN, int K>
+__global__ __launch_bounds__(256) void fwd_band_kernel(Job<real> J, real *part, int nbands) {
+  __shared__ real slots[Slots<K>::kCount * 256];
+  const int band = blockIdx.x % nbands;
+  const int s0 = ((blockIdx.x / nbands) * 64 + threadIdx.x) * K;
+  const int a = blockIdx.y * 4 + threadIdx.y;
+  const int z = blockIdx.z;
+  if (s0 >= J.Ns || a >= J.Na) return;
+  real *acc = slots + threadIdx.y * 64 + threadIdx.x;
+  const Band<real> B{band, nbands, part};
+  if (FAN)
+    fan_fwd_group<real, K, true>(J, s0, a, z, acc, B);
+  else
+    radon_fwd_group<real, K, true>(J, s0, a, z, acc, B);
+}
+
+template <typename real>
+__global__ __launch_bounds__(256) void band_reduce_kernel(Job<real> J, const real *part,
+                                                          int nbands) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s < J.Ns) band_reduce_entry(J, part, nbands, s, blockIdx.y, blockIdx.z);
+}
+
 template <typename real>
 __global__ void back_table_kernel(AngleRec<real> *__restrict__ tab, const real *__restrict__ ofs,
                                   const real *__restrict__ geo, int fan, int Na) {
@@ -208,11 +234,16 @@ static int check_sizes(const Layout &L) {
 
 static size_t elem_size(int dtype) { return dtype == GT_FLOAT64 ? 8 : 4; }
 
+// row bands of the forward for the current tuning (bands need bin groups)
+static int fwd_bands() { return tuning().fwd_group > 1 ? tuning().fwd_bands : 1; }
+
 static size_t fwd_ws(int dtype, const Layout &L, int img_order) {
   const size_t img = align_up(elem_size(dtype) * (size_t)L.Nx * L.Ny * L.Nz);
   size_t n = 0;
   if (img_order != GT_ORDER_F) n += img;                  // x-fastest copy
   if (!(img_order == GT_ORDER_C && L.Nz == 1)) n += img;  // y-fastest copy
+  const int nb = fwd_bands();
+  if (nb > 1) n += align_up(elem_size(dtype) * (size_t)nb * L.Ns * L.Na * L.Nz);  // band partials
   return n;
 }
 
@@ -267,7 +298,25 @@ static int fwd_impl(hipStream_t st, real *sino, const real *img, const real *ofs
   J.so = order_strides(sino_order, L.Ns, L.Na, L.Nz);
   const int K = tuning().fwd_group;
   const dim3 block(64, 4);
-  const dim3 grid((L.Ns + 64 * K - 1) / (64 * K), (L.Na + 3) / 4, L.Nz);
+  const int groups = (L.Ns + 64 * K - 1) / (64 * K);
+  const int nb = fwd_bands();
+  if (nb > 1) {
+    GT_REQUIRE((int64_t)groups * nb < ((int64_t)1 << 31), "too many detector groups");
+    // the band partials are the last part of the workspace (after the image copies)
+    const size_t part_bytes = align_up(sizeof(real) * (size_t)nb * L.Ns * L.Na * L.Nz);
+    real *part = reinterpret_cast<real *>(static_cast<char *>(ws) + (need - part_bytes));
+    const dim3 grid(groups * nb, (L.Na + 3) / 4, L.Nz);
+    if (K == 8)
+      hipLaunchKernelGGL((fwd_band_kernel<real, FAN, 8>), grid, block, 0, st, J, part, nb);
+    else if (K == 4)
+      hipLaunchKernelGGL((fwd_band_kernel<real, FAN, 4>), grid, block, 0, st, J, part, nb);
+    else
+      hipLaunchKernelGGL((fwd_band_kernel<real, FAN, 2>), grid, block, 0, st, J, part, nb);
+    hipLaunchKernelGGL(band_reduce_kernel<real>, dim3((L.Ns + 255) / 256, L.Na, L.Nz), dim3(256),
+                       0, st, J, part, nb);
+    return check_launch(FAN ? "fan forward projection" : "radon forward projection");
+  }
+  const dim3 grid(groups, (L.Na + 3) / 4, L.Nz);
   if (K == 8)
     hipLaunchKernelGGL((fwd_kernel<real, FAN, 8>), grid, block, 0, st, J);
   else if (K == 4)

@@ -44,6 +44,11 @@ extern "C" int gt_tune_set(int key, int value) {
         return gt::fail(GT_EINVAL, "backprojection rows per lane must be 1, 2 or 4");
       t.back_rows = value;
       return GT_OK;
+    case GT_TUNE_FWD_BANDS:
+      if (value != 1 && value != 2 && value != 4 && value != 8)
+        return gt::fail(GT_EINVAL, "forward row bands must be 1, 2, 4 or 8");
+      t.fwd_bands = value;
+      return GT_OK;
     default:
       return gt::fail(GT_EINVAL, "unknown tuning key");
   }
@@ -51,7 +56,16 @@ extern "C" int gt_tune_set(int key, int value) {
 
 extern "C" int gt_tune_get(int key) {
   const gt::Tuning &t = gt::tuning();
-  return key == GT_TUNE_FWD_GROUP ? t.fwd_group : key == GT_TUNE_BACK_ROWS ? t.back_rows : -1;
+  switch (key) {
+    case GT_TUNE_FWD_GROUP:
+      return t.fwd_group;
+    case GT_TUNE_BACK_ROWS:
+      return t.back_rows;
+    case GT_TUNE_FWD_BANDS:
+      return t.fwd_bands;
+    default:
+      return -1;
+  }
 }
 
 extern "C" size_t gt_last_error(char *buf, size_t n) {

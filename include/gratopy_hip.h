@@ -45,8 +45,13 @@ int gt_version(void);
  * driver (gratopy/gratopy.py:304-313).  Results do not depend on them beyond
  * rounding.
  *   GT_TUNE_FWD_GROUP: detector bins per lane in the forward projection (1, 2, 4, 8)
- *   GT_TUNE_BACK_ROWS: pixels per lane in the backprojection (1, 2, 4) */
-enum { GT_TUNE_FWD_GROUP = 0, GT_TUNE_BACK_ROWS = 1 };
+ *   GT_TUNE_BACK_ROWS: pixels per lane in the backprojection (1, 2, 4)
+ *   GT_TUNE_FWD_BANDS: row bands of the forward strip walk (1, 2, 4, 8; applies when
+ *                      GT_TUNE_FWD_GROUP > 1).  With B > 1 bands, consecutive thread
+ *                      blocks take different bands (so each XCD's L2 holds one band of
+ *                      the image), and the workspace grows by B * Ns * Na * Nz elements
+ *                      for the per-band partial sums. */
+enum { GT_TUNE_FWD_GROUP = 0, GT_TUNE_BACK_ROWS = 1, GT_TUNE_FWD_BANDS = 2 };
 int gt_tune_set(int key, int value);
 int gt_tune_get(int key);
 

@@ -1,5 +1,6 @@
 """Every launch variant the autotuner can pick (forward bin groups K = 1, 2,
-4, 8; backprojection rows per lane RY = 1, 2, 4) meets the parity bar against
+4, 8, with or without 8 row bands; backprojection rows per lane RY = 1, 2, 4)
+meets the parity bar against
 the oracle: normwise 1e-5 in float32, 1e-12 in float64, on regular, shifted,
 ragged and degenerate geometries (including fan bins so wide that a bin
 group's union strip is irregular and the group falls back to per-bin rays)."""
@@ -32,13 +33,13 @@ CASES = [
 def restore_variant():
     before = autotune.current_variant()
     yield
-    autotune.set_variant(before["fwd_group"], before["back_rows"])
+    autotune.set_variant(before["fwd_group"], before["back_rows"], before["fwd_bands"])
 
 
-@pytest.mark.parametrize("K", autotune.FWD_GROUPS)
-def test_forward_variants(K, restore_variant):
-    print(f"variant K={K}", flush=True)
-    autotune.set_variant(fwd_group=K)
+@pytest.mark.parametrize("K,bands", autotune.FWD_VARIANTS)
+def test_forward_variants(K, bands, restore_variant):
+    print(f"variant K={K} bands={bands}", flush=True)
+    autotune.set_variant(fwd_group=K, fwd_bands=bands)
     for geometry, shape, na, nd, kw in CASES:
         g = gratopy.RADON if geometry == "radon" else gratopy.FANBEAM
         ps = gratopy.ProjectionSettings("cuda:0", g, shape, na, nd, **kw)
@@ -47,7 +48,7 @@ def test_forward_variants(K, restore_variant):
             ref = _oracle(ps, img, dtype, False)
             got = _hip(ps, img, dtype, False)
             scale = max(np.abs(ref).max(), 1e-3)
-            assert np.abs(got - ref).max() <= TOL[dtype] * scale, (geometry, shape, K, dtype)
+            assert np.abs(got - ref).max() <= TOL[dtype] * scale, (geometry, shape, K, bands, dtype)
 
 
 @pytest.mark.parametrize("RY", autotune.BACK_ROWS)
@@ -85,21 +86,21 @@ def _large_case(geometry):
     return _LARGE[geometry]
 
 
-@pytest.mark.parametrize("K", autotune.FWD_GROUPS)
+@pytest.mark.parametrize("K,bands", autotune.FWD_VARIANTS)
 @pytest.mark.parametrize("geometry", ["radon", "fan"])
-def test_forward_variants_large_float32(geometry, K, restore_variant):
+def test_forward_variants_large_float32(geometry, K, bands, restore_variant):
     """At 768 detectors the float32 result of every variant is within 1e-5 of
     the float64 oracle, or within 1.2x the distance of the reference
     algorithm in float32 (the oracle in float32).  At this size both sit at
     the float32 floor (~1.4e-5 for Radon): the float32 geometry tables alone
     put ~3e-5 detector bins of error into x*X + y*Y + off."""
-    print(f"variant K={K} large {geometry}", flush=True)
+    print(f"variant K={K} bands={bands} large {geometry}", flush=True)
     ps, img, truth, ref32 = _large_case(geometry)
-    autotune.set_variant(fwd_group=K)
+    autotune.set_variant(fwd_group=K, fwd_bands=bands)
     got = _hip(ps, img, F32, False)
     e_hip, e_ref = rel_err(got, truth), rel_err(ref32, truth)
-    print(f"K={K} {geometry}: float32 error vs float64 oracle: hip {e_hip:.3e}, reference algorithm "
-          f"{e_ref:.3e}", flush=True)
+    print(f"K={K} bands={bands} {geometry}: float32 error vs float64 oracle: hip {e_hip:.3e}, "
+          f"reference algorithm {e_ref:.3e}", flush=True)
     assert e_hip <= max(TOL[F32], 1.2 * e_ref)
 
 
@@ -113,12 +114,13 @@ def test_autotuner_picks_a_variant_and_caches_it(tmp_path, monkeypatch, restore_
     sino = gratopy.forwardprojection(img, ps)
     best = autotune.tune(ps, img.clone(), sino.clone(), repeats=2)
     assert best["fwd_group"] in autotune.FWD_GROUPS and best["back_rows"] in autotune.BACK_ROWS
-    assert autotune.current_variant() == {"fwd_group": best["fwd_group"], "back_rows": best["back_rows"]}
-    autotune.set_variant(1, 1)
+    assert autotune.current_variant() == {"fwd_group": best["fwd_group"], "back_rows": best["back_rows"],
+                                          "fwd_bands": best["fwd_bands"]}
+    autotune.set_variant(1, 1, 1)
     assert autotune.apply(ps, F32, 1)
     assert autotune.current_variant()["fwd_group"] == best["fwd_group"]
     # all variants agree to rounding on the tuned geometry
     ref = dev.to_host(gratopy.forwardprojection(img, ps))
-    for K in autotune.FWD_GROUPS:
-        autotune.set_variant(fwd_group=K)
+    for K, bands in autotune.FWD_VARIANTS:
+        autotune.set_variant(fwd_group=K, fwd_bands=bands)
         assert rel_err(dev.to_host(gratopy.forwardprojection(img, ps)), ref) < 1e-5
