@@ -649,13 +649,47 @@ __device__ __forceinline__ void stage_entry(real *__restrict__ S, const real *__
 // taken in padded-sinogram indices (bin s at s + 2) and clamped to
 // [0.5, Ns + 2.5], where the two taps floor(p), floor(p) + 1 stay inside the
 // padded row and truncation equals floor.
-template <typename real, bool FAN, int RY>
+//
+// BUF: the two taps are read with one buffer load through a descriptor of
+// the slice's padded sinogram (per-angle row offset in a scalar register,
+// 32-bit per-lane offset), which saves the 64-bit address arithmetic; needs
+// Na * (Ns + 4) * sizeof(real) < 2^31 (buffer_fits).  Offsets outside the
+// descriptor would read 0 rather than fault; the clamp keeps them inside.
+template <typename real>
+struct Taps {
+  real v0, v1;
+};
+
+template <typename real>
+__device__ __forceinline__ Taps<real> buffer_taps(__amdgpu_buffer_rsrc_t rs, int voff, int soff) {
+  Taps<real> t;
+  if constexpr (sizeof(real) == 4) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    const f2 v = __builtin_bit_cast(f2, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0));
+    t.v0 = v.x;
+    t.v1 = v.y;
+  } else {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const d2 v = __builtin_bit_cast(d2, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0));
+    t.v0 = v.x;
+    t.v1 = v.y;
+  }
+  return t;
+}
+
+inline bool buffer_fits(size_t elem, int Ns, int Na) {
+  return (size_t)Na * (size_t)(Ns + 4) * elem < ((size_t)1 << 31);
+}
+
+template <typename real, bool FAN, int RY, bool BUF = false>
 __device__ __forceinline__ void back_pixels(const Job<real> &J, int x, int y0, int z) {
   using A = Arith<real>;
   const real xr = (real)x, yr = (real)y0;
   const int Ns = J.Ns, Na = J.Na;
   const int Nsp = Ns + 4;
   const real *Sz = J.S + (int64_t)z * Na * Nsp;
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)Sz, (short)0, BUF ? (int)((size_t)Na * Nsp * sizeof(real)) : 0, 0x00020000);
   const AngleRec<real> *__restrict__ tab = J.tab;
   const real lo = (real)0.5, hi = (real)Ns + (real)2.5;
   const real mid_det = FAN ? J.geo[5] + (real)2 : (real)0;
@@ -678,9 +712,18 @@ __device__ __forceinline__ void back_pixels(const Job<real> &J, int x, int y0, i
         p = q;
       }
       p = A::clamp(p, lo, hi);
-      const real *tap = row + (int)p;
       const real f = A::fract(p);
-      const real v0 = tap[0], v1 = tap[1];
+      real v0, v1;
+      if (BUF) {
+        const Taps<real> tp = buffer_taps<real>(rs, (int)p * (int)sizeof(real),
+                                                a * Nsp * (int)sizeof(real));
+        v0 = tp.v0;
+        v1 = tp.v1;
+      } else {
+        const real *tap = row + (int)p;
+        v0 = tap[0];
+        v1 = tap[1];
+      }
       const real v = v0 + f * (v1 - v0);
       acc[r] = FAN ? acc[r] + v * w : acc[r] + v;
       if (FAN) {

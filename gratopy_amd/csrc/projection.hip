@@ -100,12 +100,20 @@ __global__ __launch_bounds__(256) void stage_sino_kernel(real *__restrict__ S,
   if (j < Ns + 4) stage_entry(S, sino, si, ofs, sdpd, geo, fan, Ns, Na, j, blockIdx.y, blockIdx.z);
 }
 
-template <typename real, bool FAN, int RY>
+template <typename real, bool FAN, int RY, bool BUF>
 __global__ __launch_bounds__(256) void back_kernel(Job<real> J) {
   const int x = blockIdx.x * 16 + (threadIdx.x & 15);
   const int y0 = (blockIdx.y * 16 + (threadIdx.x >> 4)) * RY;
   if (x >= J.Nx || y0 >= J.Ny) return;
-  back_pixels<real, FAN, RY>(J, x, y0, blockIdx.z);
+  back_pixels<real, FAN, RY, BUF>(J, x, y0, blockIdx.z);
+}
+
+template <typename real, bool FAN, int RY>
+static void launch_back(hipStream_t st, const Job<real> &J, dim3 grid) {
+  if (buffer_fits(sizeof(real), J.Ns, J.Na))
+    hipLaunchKernelGGL((back_kernel<real, FAN, RY, true>), grid, dim3(256), 0, st, J);
+  else
+    hipLaunchKernelGGL((back_kernel<real, FAN, RY, false>), grid, dim3(256), 0, st, J);
 }
 
 // ---------------------------------------------------------------------------
@@ -351,11 +359,11 @@ static int back_impl(hipStream_t st, real *img, const real *sino, const real *of
   const int RY = tuning().back_rows;
   const dim3 grid((L.Nx + 15) / 16, (L.Ny + 16 * RY - 1) / (16 * RY), L.Nz);
   if (RY == 4)
-    hipLaunchKernelGGL((back_kernel<real, FAN, 4>), grid, dim3(256), 0, st, J);
+    launch_back<real, FAN, 4>(st, J, grid);
   else if (RY == 2)
-    hipLaunchKernelGGL((back_kernel<real, FAN, 2>), grid, dim3(256), 0, st, J);
+    launch_back<real, FAN, 2>(st, J, grid);
   else
-    hipLaunchKernelGGL((back_kernel<real, FAN, 1>), grid, dim3(256), 0, st, J);
+    launch_back<real, FAN, 1>(st, J, grid);
   return check_launch(FAN ? "fan backprojection" : "radon backprojection");
 }
 
