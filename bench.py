@@ -44,6 +44,7 @@ def parse():
     p.add_argument("--fwd-group", type=int, default=None, help="detector bins per lane (forward)")
     p.add_argument("--back-rows", type=int, default=None, help="pixels per lane (backprojection)")
     p.add_argument("--fwd-bands", type=int, default=None, help="row bands of the forward (1 or 8)")
+    p.add_argument("--back-bands", type=int, default=None, help="angle bands of the backprojection")
     p.add_argument("--no-batched", action="store_true",
                    help="skip the batched-vs-plain solver measurements")
     p.add_argument("--tv-iterations", type=int, default=6,
@@ -257,6 +258,8 @@ def main():
         _hip.check(lib.gt_tune_set(1, args.back_rows), "gt_tune_set")
     if args.fwd_bands is not None:
         _hip.check(lib.gt_tune_set(2, args.fwd_bands), "gt_tune_set")
+    if args.back_bands is not None:
+        _hip.check(lib.gt_tune_set(3, args.back_bands), "gt_tune_set")
     Nz, N, Na, Ns, RE, R, iw, dw, n_ell = WORKLOADS[args.workload]
     ps_kwargs = dict(R=R, RE=RE, image_width=iw, detector_width=dw)
     device = "cuda:0"
@@ -269,7 +272,7 @@ def main():
     stream = torch.cuda.current_stream()
     tuning_source = "fixed"
     tuned = None
-    if args.fwd_group is None and args.back_rows is None and args.fwd_bands is None:
+    if all(v is None for v in (args.fwd_group, args.back_rows, args.fwd_bands, args.back_bands)):
         from gratopy_amd import autotune
 
         back.copy_(img)
@@ -333,7 +336,7 @@ def main():
                    "detectors": Ns, "RE": RE, "R": R, "image_width": iw, "detector_width": dw,
                    "order": "F"},
         "tuning": {"fwd_group": lib.gt_tune_get(0), "fwd_bands": lib.gt_tune_get(2),
-                   "back_rows": lib.gt_tune_get(1),
+                   "back_rows": lib.gt_tune_get(1), "back_bands": lib.gt_tune_get(3),
                    "source": tuning_source,
                    "trial_ms": None if tuned is None else {"fwd": tuned["fwd_ms"], "back": tuned["back_ms"]}},
         "fwd_ms": round(fwd_med * 1e3, 3),

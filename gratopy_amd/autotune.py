@@ -3,7 +3,8 @@
 The forward kernels assign K adjacent detector bins to a lane (K in 1, 2,
 4, 8; K = 1 is the per-ray kernel), optionally split the strip walk into 8
 row bands (one per XCD, partial sums added in band order), and the
-backprojection assigns RY pixels along y to a lane (RY in 1, 2, 4).
+backprojection assigns RY pixels along y to a lane (RY in 1, 2, 4),
+optionally over 8 angle bands on the same pattern.
 Batched launches use K and RY only.  :func:`tune` times every variant on the
 caller's own geometry and array shapes with HIP events, keeps the fastest,
 and remembers it per shape key in a JSON cache
@@ -29,7 +30,9 @@ BACK_ROWS = (1, 2, 4)
 FWD_BANDS = (1, 8)
 # forward variants: (bins per lane, row bands); bands need bin groups
 FWD_VARIANTS = tuple((k, 1) for k in FWD_GROUPS) + tuple((k, 8) for k in FWD_GROUPS if k > 1)
-KEY_FWD, KEY_BACK, KEY_BANDS = 0, 1, 2
+# backprojection variants: (rows per lane, angle bands)
+BACK_VARIANTS = tuple((r, 1) for r in BACK_ROWS) + tuple((r, 8) for r in BACK_ROWS)
+KEY_FWD, KEY_BACK, KEY_BANDS, KEY_BACK_BANDS = 0, 1, 2, 3
 
 _lock = threading.Lock()
 
@@ -69,7 +72,7 @@ def _store(table: dict) -> None:
 
 
 def set_variant(fwd_group: int | None = None, back_rows: int | None = None,
-                fwd_bands: int | None = None) -> None:
+                fwd_bands: int | None = None, back_bands: int | None = None) -> None:
     lib = _hip.load_library()
     if fwd_group is not None:
         _hip.check(lib.gt_tune_set(KEY_FWD, int(fwd_group)), "gt_tune_set")
@@ -77,16 +80,22 @@ def set_variant(fwd_group: int | None = None, back_rows: int | None = None,
         _hip.check(lib.gt_tune_set(KEY_BACK, int(back_rows)), "gt_tune_set")
     if fwd_bands is not None:
         _hip.check(lib.gt_tune_set(KEY_BANDS, int(fwd_bands)), "gt_tune_set")
+    if back_bands is not None:
+        _hip.check(lib.gt_tune_set(KEY_BACK_BANDS, int(back_bands)), "gt_tune_set")
 
 
 def current_variant() -> dict:
     lib = _hip.load_library()
     return {"fwd_group": lib.gt_tune_get(KEY_FWD), "back_rows": lib.gt_tune_get(KEY_BACK),
-            "fwd_bands": lib.gt_tune_get(KEY_BANDS)}
+            "fwd_bands": lib.gt_tune_get(KEY_BANDS), "back_bands": lib.gt_tune_get(KEY_BACK_BANDS)}
 
 
 def _fwd_key(k: int, b: int) -> str:
     return f"K{k}" if b == 1 else f"K{k}xB{b}"
+
+
+def _back_key(r: int, b: int) -> str:
+    return f"RY{r}" if b == 1 else f"RY{r}xB{b}"
 
 
 def _time(fn, repeats: int) -> float:
@@ -118,20 +127,22 @@ def tune(projectionsetting, img, sino, repeats: int = 3, store: bool = True) -> 
         set_variant(fwd_group=k, fwd_bands=b)
         fwd_times[(k, b)] = _time(lambda: forwardprojection(img, ps, sino=sino), repeats)
     back_times = {}
-    for r in BACK_ROWS:
-        set_variant(back_rows=r)
-        back_times[r] = _time(lambda: backprojection(sino, ps, img=img), repeats)
+    for r, b in BACK_VARIANTS:
+        set_variant(back_rows=r, back_bands=b)
+        back_times[(r, b)] = _time(lambda: backprojection(sino, ps, img=img), repeats)
     k, b = min(fwd_times, key=fwd_times.get)
-    best = {"fwd_group": k, "fwd_bands": b, "back_rows": min(back_times, key=back_times.get),
-            "fwd_ms": {_fwd_key(*v): t for v, t in fwd_times.items()}, "back_ms": back_times}
-    set_variant(best["fwd_group"], best["back_rows"], best["fwd_bands"])
+    r, rb = min(back_times, key=back_times.get)
+    best = {"fwd_group": k, "fwd_bands": b, "back_rows": r, "back_bands": rb,
+            "fwd_ms": {_fwd_key(*v): t for v, t in fwd_times.items()},
+            "back_ms": {_back_key(*v): t for v, t in back_times.items()}}
+    set_variant(best["fwd_group"], best["back_rows"], best["fwd_bands"], best["back_bands"])
     if store:
         nz = int(img.shape[2]) if img.dim() > 2 else 1
         with _lock:
             table = _load()
             table[shape_key(ps, dev.to_np_dtype(img.dtype), nz)] = {
                 "fwd_group": best["fwd_group"], "fwd_bands": best["fwd_bands"],
-                "back_rows": best["back_rows"]}
+                "back_rows": best["back_rows"], "back_bands": best["back_bands"]}
             _store(table)
     return best
 
@@ -139,12 +150,12 @@ def tune(projectionsetting, img, sino, repeats: int = 3, store: bool = True) -> 
 def tune_batch(settings, imgs, sinos, repeats: int = 2) -> dict:
     """Time all variants on a batch (the tile lists depend on the variant, so
     one plan is built per variant) and install the fastest.  ``imgs`` and
-    ``sinos`` are overwritten.  Batched launches walk whole strips, so the
-    row bands are switched off: single-slice calls then run the same kernels
-    as the batch."""
+    ``sinos`` are overwritten.  Batched launches use no bands, so the bands
+    are switched off: single-slice calls then run the same kernels as the
+    batch."""
     from .batch import BatchPlan
 
-    set_variant(fwd_bands=1)
+    set_variant(fwd_bands=1, back_bands=1)
 
     def timed(op):
         plan = BatchPlan(settings, imgs, sinos)
@@ -171,5 +182,6 @@ def apply(projectionsetting, dtype, nz: int = 1) -> bool:
     entry = _load().get(shape_key(projectionsetting, dtype, nz))
     if not entry:
         return False
-    set_variant(entry.get("fwd_group"), entry.get("back_rows"), entry.get("fwd_bands", 1))
+    set_variant(entry.get("fwd_group"), entry.get("back_rows"), entry.get("fwd_bands", 1),
+                entry.get("back_bands", 1))
     return True

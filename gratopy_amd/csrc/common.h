@@ -26,6 +26,7 @@ struct Tuning {
   int fwd_group = 4;  // detector bins per lane in the forward kernels (1, 2, 4, 8)
   int back_rows = 1;  // pixels per lane (along y) in the backprojection (1, 2, 4)
   int fwd_bands = 1;  // row bands of the forward strip walk (1, 2, 4, 8; used when fwd_group > 1)
+  int back_bands = 1;  // angle bands of the backprojection (1, 2, 4, 8)
 };
 Tuning &tuning();
 

@@ -681,9 +681,14 @@ inline bool buffer_fits(size_t elem, int Ns, int Na) {
   return (size_t)Na * (size_t)(Ns + 4) * elem < ((size_t)1 << 31);
 }
 
+// Angles [a0, a1) only; with part != nullptr the raw sums go to
+// part[((band*Nz + z)*Ny + y)*Nx + x] (band_back_reduce_entry adds the bands
+// in order and applies the epilogue), otherwise they are stored.
 template <typename real, bool FAN, int RY, bool BUF = false>
-__device__ __forceinline__ void back_pixels(const Job<real> &J, int x, int y0, int z) {
+__device__ __forceinline__ void back_pixels(const Job<real> &J, int x, int y0, int z, int a0 = 0,
+                                            int a1 = -1, real *part = nullptr, int band = 0) {
   using A = Arith<real>;
+  if (a1 < 0) a1 = J.Na;
   const real xr = (real)x, yr = (real)y0;
   const int Ns = J.Ns, Na = J.Na;
   const int Nsp = Ns + 4;
@@ -697,7 +702,7 @@ __device__ __forceinline__ void back_pixels(const Job<real> &J, int x, int y0, i
 #pragma unroll
   for (int r = 0; r < RY; ++r) acc[r] = 0;
 #pragma unroll 2
-  for (int a = 0; a < Na; ++a) {
+  for (int a = a0; a < a1; ++a) {
     const AngleRec<real> &c = tab[a];
     const real *row = Sz + (int64_t)a * Nsp;
     real t = FAN ? fma(c.c[0], xr, fma(c.c[1], yr, c.c[2])) : (real)1;
@@ -737,8 +742,24 @@ __device__ __forceinline__ void back_pixels(const Job<real> &J, int x, int y0, i
 #pragma unroll
   for (int r = 0; r < RY; ++r) {
     const int y = y0 + r;
-    if (y < J.Ny) store_back(J, x, y, z, acc[r]);
+    if (y >= J.Ny) break;
+    if (part)
+      part[(((int64_t)band * J.Nz + z) * J.Ny + y) * J.Nx + x] = acc[r];
+    else
+      store_back(J, x, y, z, acc[r]);
   }
+}
+
+// Sum of the angle bands of pixel (x, y, z) in band order, then the epilogue.
+template <typename real>
+__device__ __forceinline__ void band_back_reduce_entry(const Job<real> &J,
+                                                       const real *__restrict__ part, int count,
+                                                       int x, int y, int z) {
+  const int64_t stride = (int64_t)J.Nz * J.Ny * J.Nx;
+  const real *p = part + ((int64_t)z * J.Ny + y) * J.Nx + x;
+  real acc = p[0];
+  for (int b = 1; b < count; ++b) acc += p[b * stride];
+  store_back(J, x, y, z, acc);
 }
 
 }  // namespace gt

@@ -108,12 +108,46 @@ __global__ __launch_bounds__(256) void back_kernel(Job<real> J) {
   back_pixels<real, FAN, RY, BUF>(J, x, y0, blockIdx.z);
 }
 
+// Banded form: blockIdx.x = band + nbands * column block; band b sums the
+// angles [b*Na/nbands, (b+1)*Na/nbands), so each XCD's L2 serves the padded
+// sinogram rows of one angle band.
+template <typename real, bool FAN, int RY, bool BUF>
+__global__ __launch_bounds__(256) void back_band_kernel(Job<real> J, real *part, int nbands) {
+  const int band = blockIdx.x % nbands;
+  const int x = (blockIdx.x / nbands) * 16 + (threadIdx.x & 15);
+  const int y0 = (blockIdx.y * 16 + (threadIdx.x >> 4)) * RY;
+  if (x >= J.Nx || y0 >= J.Ny) return;
+  const int a0 = (int)((int64_t)J.Na * band / nbands);
+  const int a1 = (int)((int64_t)J.Na * (band + 1) / nbands);
+  back_pixels<real, FAN, RY, BUF>(J, x, y0, blockIdx.z, a0, a1, part, band);
+}
+
+template <typename real>
+__global__ __launch_bounds__(256) void back_reduce_kernel(Job<real> J, const real *part,
+                                                          int nbands) {
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x < J.Nx && y < J.Ny) band_back_reduce_entry(J, part, nbands, x, y, blockIdx.z);
+}
+
 template <typename real, bool FAN, int RY>
-static void launch_back(hipStream_t st, const Job<real> &J, dim3 grid) {
-  if (buffer_fits(sizeof(real), J.Ns, J.Na))
+static void launch_back(hipStream_t st, const Job<real> &J, int nbands, real *part) {
+  const dim3 grid(((J.Nx + 15) / 16) * nbands, (J.Ny + 16 * RY - 1) / (16 * RY), J.Nz);
+  const bool buf = buffer_fits(sizeof(real), J.Ns, J.Na);
+  if (nbands > 1) {
+    if (buf)
+      hipLaunchKernelGGL((back_band_kernel<real, FAN, RY, true>), grid, dim3(256), 0, st, J, part,
+                         nbands);
+    else
+      hipLaunchKernelGGL((back_band_kernel<real, FAN, RY, false>), grid, dim3(256), 0, st, J, part,
+                         nbands);
+    hipLaunchKernelGGL(back_reduce_kernel<real>, dim3((J.Nx + 15) / 16, (J.Ny + 15) / 16, J.Nz),
+                       dim3(256), 0, st, J, part, nbands);
+  } else if (buf) {
     hipLaunchKernelGGL((back_kernel<real, FAN, RY, true>), grid, dim3(256), 0, st, J);
-  else
+  } else {
     hipLaunchKernelGGL((back_kernel<real, FAN, RY, false>), grid, dim3(256), 0, st, J);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -255,9 +289,14 @@ static size_t fwd_ws(int dtype, const Layout &L, int img_order) {
   return n;
 }
 
+static size_t back_partials(int dtype, const Layout &L) {
+  const int nb = tuning().back_bands;
+  return nb > 1 ? align_up(elem_size(dtype) * (size_t)nb * L.Nx * L.Ny * L.Nz) : 0;
+}
+
 static size_t back_ws(int dtype, const Layout &L) {
   return align_up(elem_size(dtype) * 8 * (size_t)L.Na) +
-         align_up(elem_size(dtype) * (size_t)(L.Ns + 4) * L.Na * L.Nz);
+         align_up(elem_size(dtype) * (size_t)(L.Ns + 4) * L.Na * L.Nz) + back_partials(dtype, L);
 }
 
 template <typename real>
@@ -357,13 +396,17 @@ static int back_impl(hipStream_t st, real *img, const real *sino, const real *of
   J.S = S;
   J.tab = tab;
   const int RY = tuning().back_rows;
-  const dim3 grid((L.Nx + 15) / 16, (L.Ny + 16 * RY - 1) / (16 * RY), L.Nz);
+  const int nb = tuning().back_bands;
+  GT_REQUIRE((int64_t)((L.Nx + 15) / 16) * nb < ((int64_t)1 << 31), "image too wide");
+  // the band partials are the last part of the workspace
+  const size_t part_bytes = back_partials(sizeof(real) == 8 ? GT_FLOAT64 : GT_FLOAT32, L);
+  real *part = reinterpret_cast<real *>(w + (need - part_bytes));
   if (RY == 4)
-    launch_back<real, FAN, 4>(st, J, grid);
+    launch_back<real, FAN, 4>(st, J, nb, part);
   else if (RY == 2)
-    launch_back<real, FAN, 2>(st, J, grid);
+    launch_back<real, FAN, 2>(st, J, nb, part);
   else
-    launch_back<real, FAN, 1>(st, J, grid);
+    launch_back<real, FAN, 1>(st, J, nb, part);
   return check_launch(FAN ? "fan backprojection" : "radon backprojection");
 }
 

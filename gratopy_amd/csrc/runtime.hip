@@ -49,6 +49,11 @@ extern "C" int gt_tune_set(int key, int value) {
         return gt::fail(GT_EINVAL, "forward row bands must be 1, 2, 4 or 8");
       t.fwd_bands = value;
       return GT_OK;
+    case GT_TUNE_BACK_BANDS:
+      if (value != 1 && value != 2 && value != 4 && value != 8)
+        return gt::fail(GT_EINVAL, "backprojection angle bands must be 1, 2, 4 or 8");
+      t.back_bands = value;
+      return GT_OK;
     default:
       return gt::fail(GT_EINVAL, "unknown tuning key");
   }
@@ -63,6 +68,8 @@ extern "C" int gt_tune_get(int key) {
       return t.back_rows;
     case GT_TUNE_FWD_BANDS:
       return t.fwd_bands;
+    case GT_TUNE_BACK_BANDS:
+      return t.back_bands;
     default:
       return -1;
   }

@@ -50,8 +50,11 @@ int gt_version(void);
  *                      GT_TUNE_FWD_GROUP > 1).  With B > 1 bands, consecutive thread
  *                      blocks take different bands (so each XCD's L2 holds one band of
  *                      the image), and the workspace grows by B * Ns * Na * Nz elements
- *                      for the per-band partial sums. */
-enum { GT_TUNE_FWD_GROUP = 0, GT_TUNE_BACK_ROWS = 1, GT_TUNE_FWD_BANDS = 2 };
+ *                      for the per-band partial sums.
+ *   GT_TUNE_BACK_BANDS: angle bands of the backprojection (1, 2, 4, 8), on the same
+ *                      pattern (each XCD's L2 holds the padded sinogram rows of one band;
+ *                      the workspace grows by B * Nx * Ny * Nz elements). */
+enum { GT_TUNE_FWD_GROUP = 0, GT_TUNE_BACK_ROWS = 1, GT_TUNE_FWD_BANDS = 2, GT_TUNE_BACK_BANDS = 3 };
 int gt_tune_set(int key, int value);
 int gt_tune_get(int key);
 

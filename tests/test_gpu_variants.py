@@ -1,5 +1,6 @@
 """Every launch variant the autotuner can pick (forward bin groups K = 1, 2,
-4, 8, with or without 8 row bands; backprojection rows per lane RY = 1, 2, 4)
+4, 8, with or without 8 row bands; backprojection rows per lane RY = 1, 2, 4,
+with or without 8 angle bands)
 meets the parity bar against
 the oracle: normwise 1e-5 in float32, 1e-12 in float64, on regular, shifted,
 ragged and degenerate geometries (including fan bins so wide that a bin
@@ -33,7 +34,8 @@ CASES = [
 def restore_variant():
     before = autotune.current_variant()
     yield
-    autotune.set_variant(before["fwd_group"], before["back_rows"], before["fwd_bands"])
+    autotune.set_variant(before["fwd_group"], before["back_rows"], before["fwd_bands"],
+                         before["back_bands"])
 
 
 @pytest.mark.parametrize("K,bands", autotune.FWD_VARIANTS)
@@ -51,10 +53,10 @@ def test_forward_variants(K, bands, restore_variant):
             assert np.abs(got - ref).max() <= TOL[dtype] * scale, (geometry, shape, K, bands, dtype)
 
 
-@pytest.mark.parametrize("RY", autotune.BACK_ROWS)
-def test_backward_variants(RY, restore_variant):
-    print(f"variant RY={RY}", flush=True)
-    autotune.set_variant(back_rows=RY)
+@pytest.mark.parametrize("RY,bands", autotune.BACK_VARIANTS)
+def test_backward_variants(RY, bands, restore_variant):
+    print(f"variant RY={RY} bands={bands}", flush=True)
+    autotune.set_variant(back_rows=RY, back_bands=bands)
     for geometry, shape, na, nd, kw in CASES:
         g = gratopy.RADON if geometry == "radon" else gratopy.FANBEAM
         ps = gratopy.ProjectionSettings("cuda:0", g, shape, na, nd, **kw)
@@ -63,7 +65,7 @@ def test_backward_variants(RY, restore_variant):
             ref = _oracle(ps, sino, dtype, True)
             got = _hip(ps, sino, dtype, True, in_order="C", out_order="C")
             scale = max(np.abs(ref).max(), 1e-3)
-            assert np.abs(got - ref).max() <= TOL[dtype] * scale, (geometry, shape, RY, dtype)
+            assert np.abs(got - ref).max() <= TOL[dtype] * scale, (geometry, shape, RY, bands, dtype)
 
 
 _LARGE = {}
@@ -115,8 +117,8 @@ def test_autotuner_picks_a_variant_and_caches_it(tmp_path, monkeypatch, restore_
     best = autotune.tune(ps, img.clone(), sino.clone(), repeats=2)
     assert best["fwd_group"] in autotune.FWD_GROUPS and best["back_rows"] in autotune.BACK_ROWS
     assert autotune.current_variant() == {"fwd_group": best["fwd_group"], "back_rows": best["back_rows"],
-                                          "fwd_bands": best["fwd_bands"]}
-    autotune.set_variant(1, 1, 1)
+                                          "fwd_bands": best["fwd_bands"], "back_bands": best["back_bands"]}
+    autotune.set_variant(1, 1, 1, 1)
     assert autotune.apply(ps, F32, 1)
     assert autotune.current_variant()["fwd_group"] == best["fwd_group"]
     # all variants agree to rounding on the tuned geometry
