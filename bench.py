@@ -303,10 +303,12 @@ def main():
     work = N * N * Na * Nz
     gups = work / step / 1e9
 
-    # dominant launch: the slower of the two directions; algorithmic bytes =
+    # dominant launch: the slower of the two directions (all kernels of that
+    # direction: image transpose or sinogram staging, the projection kernel
+    # and, when banded, the ordered band reduction); algorithmic bytes =
     # 2 interpolation taps x 4 B per pixel-angle per direction
     fwd_med, back_med = float(np.median(t_fwd)), float(np.median(t_back))
-    dom_name, dom_t = ("fwd_kernel", fwd_med) if fwd_med >= back_med else ("back_kernel", back_med)
+    dom_name, dom_t = ("forward", fwd_med) if fwd_med >= back_med else ("backprojection", back_med)
     algo_bytes = 8.0 * work
     achieved = algo_bytes / dom_t
 
