@@ -470,6 +470,7 @@ __device__ __forceinline__ void fan_fwd_group(const Job<real> &J, int s0, int a,
     const real xlow = F.qx + F.dmx * (u - F.qy);
     const int xh = (int)floor(fmax(fmin(xcap, xhigh) + F.midx, (real)-1));
     const int xl = (int)ceil(fmin(fmax(-F.midx, xlow) + F.midx, (real)F.Nxx));
+    if (xl > xh) continue;
     const double xr = (double)xl - (double)F.midx, ud = (double)y - (double)F.midy;
     const double t0 = (double)F.ddx * xr + (double)F.ddy * ud + (double)F.RE;
     real t = (real)t0;
@@ -549,15 +550,16 @@ __device__ __forceinline__ void radon_fwd_group(const Job<real> &J, int s0, int 
     }
     xl = xl < 0 ? 0 : xl;
     xh = xh > Nxx - 1 ? Nxx - 1 : xh;
+    if (xl > xh) continue;
     // detector position of (xl, y) relative to bin s0 - 2 (the slot index),
     // evaluated in double: x*X and y*Y + off are up to Ns in size and nearly
     // cancel; inside the strip it lies in [1, K+2], so truncation is floor
     real pr = (real)((double)xl * (double)X + (double)y * (double)Y + (double)off - (double)s0 + 2.0);
     const real *row = base + (int64_t)y * row_stride;
     for (int x = xl; x <= xh; ++x) {
+      const real v = row[x];
       const real p = A::clamp(pr, lo, hi);
       const real f = A::fract(p);
-      const real v = row[x];
       real *slot = acc + (int)p * 256;
       slot[0] += v - f * v;
       slot[256] += f * v;
