@@ -77,6 +77,15 @@ PROTOTYPES: dict[str, tuple] = {
     "gt_batch_plan_destroy": (_i, [_vp]),
     "gt_batch_plan_bytes": (_sz, [_vp]),
     "gt_batch_launch": (_i, [_vp, _vp, _i, _d]),
+    "gt_plugin_compile": (_i, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                               ctypes.POINTER(ctypes.c_char_p), _i, _vp, _sz,
+                               ctypes.POINTER(_sz)]),
+    "gt_plugin_device_arch": (_i, [ctypes.c_char_p, _sz]),
+    "gt_plugin_module_load": (_i, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
+    "gt_plugin_module_unload": (_i, [_vp]),
+    "gt_plugin_function": (_i, [_vp, ctypes.c_char_p, ctypes.POINTER(_vp)]),
+    "gt_plugin_launch": (_i, [_vp, _vp, ctypes.POINTER(ctypes.c_uint),
+                              ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(_vp)]),
 }
 
 # operations of gt_batch_launch
@@ -143,7 +152,7 @@ def load_library() -> ctypes.CDLL:
 
 def last_error() -> str:
     lib = load_library()
-    buf = ctypes.create_string_buffer(1024)
+    buf = ctypes.create_string_buffer(lib.gt_last_error(None, 0) + 1)
     lib.gt_last_error(buf, len(buf))
     return buf.value.decode(errors="replace")
 

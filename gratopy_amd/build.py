@@ -16,7 +16,10 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libgratopy_hip.so")
 ARCH = os.environ.get("GRATOPY_HIP_ARCH", "gfx950")
 
-SOURCES = ("runtime.hip", "elementwise.hip", "projection.hip", "batch.hip")
+SOURCES = ("runtime.hip", "elementwise.hip", "projection.hip", "batch.hip", "plugin.hip")
+
+# hiprtc compiles operator kernels supplied at run time (plugin.hip)
+LIBS = ["-lhiprtc"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",
@@ -42,7 +45,7 @@ def build_library(verbose: bool = False) -> str:
     """Compile the HIP sources (always, even if a library exists)."""
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = OUT + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, *srcs]
+    cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, *srcs, *LIBS]
     if verbose:
         print(" ".join(cmd), flush=True)
     proc = subprocess.run(cmd, capture_output=True, text=True)

@@ -100,7 +100,19 @@ class _ProjectionOperator(_HipOperator):
             self._device_struct[key] = tables
         return self._device_struct[key]
 
+    def _kernel_arguments(self, output, argument, queue: dev.Queue) -> tuple:
+        """Geometry tables passed to a plugin projection kernel after (out, in):
+        (ofs, geo) for Radon (projection.py:372-379), (ofs, sdpd, geo) for
+        Fanbeam (gratopy.py:628-638)."""
+        tables = self._ensure_device_struct(queue, dev.to_np_dtype(argument.dtype))
+        if self._geometry_is_fan:
+            return tables["ofs"], tables["sdpd"], tables["geo"]
+        return tables["ofs"], tables["geo"]
+
     def _launch(self, output, argument, queue: dev.Queue) -> None:
+        if not self.kernel_spec.is_builtin:
+            super()._launch(output, argument, queue)
+            return
         tables = self._ensure_device_struct(queue, dev.to_np_dtype(argument.dtype))
         sino_shape = (self.detectors.number, len(self.angles))
         launch_projection(self._geometry_is_fan, self.adjoint, output, argument, tables,

@@ -271,6 +271,34 @@ size_t gt_batch_plan_bytes(const gt_batch_plan *plan);
 /* Asynchronous on `stream`; capturable in a HIP graph (kernel launches only). */
 int gt_batch_launch(gt_batch_plan *plan, gt_stream_t stream, int op, double sigma);
 
+/* ---- operator kernels compiled at run time ---------------------------------
+ * Replace the OpenCL program build, kernel lookup and launch of the operator
+ * interface (gratopy/operator/opencl.py:262-305 _get_program /
+ * _get_projection_kernel, 444-454 _invoke_kernel).  A plugin source is HIP C++
+ * whose kernels have C linkage and the names
+ * {base}[_ad]_{float|double}_{out_order}{in_order}; their arguments are
+ * (out*, in*, extra..., gt_range) where gt_range (gratopy_amd/csrc/gratopy_plugin.h)
+ * is the output shape, the reference's global size.
+ *
+ * gt_plugin_compile: hiprtc compile of `source` for `arch` (e.g. "gfx950"),
+ * with -O3 and packed FP32 instructions off ahead of `options`; needs no GPU.
+ * The code object goes to `image` if `capacity` suffices; `*image_size` always
+ * receives its size.  Compile errors carry the compiler log in gt_last_error. */
+int gt_plugin_compile(const char *source, const char *name, const char *arch,
+                      const char *const *options, int n_options, void *image, size_t capacity,
+                      size_t *image_size);
+/* Architecture name of the current device, e.g. "gfx950:sramecc+:xnack-". */
+int gt_plugin_device_arch(char *buf, size_t n);
+/* Load / unload a code object on the current device (unload synchronizes it). */
+int gt_plugin_module_load(const void *image, void **module);
+int gt_plugin_module_unload(void *module);
+/* Kernel of a loaded module by name. */
+int gt_plugin_function(void *module, const char *name, void **function);
+/* Launch grid[0..2] blocks of block[0..2] threads on `stream`; args[i] points
+ * to the value of kernel argument i. */
+int gt_plugin_launch(gt_stream_t stream, void *function, const unsigned *grid,
+                     const unsigned *block, void **args);
+
 #ifdef __cplusplus
 }
 #endif
