@@ -21,7 +21,6 @@ from __future__ import annotations
 
 import ctypes
 import hashlib
-import os
 import struct
 from dataclasses import dataclass
 from pathlib import Path
@@ -127,7 +126,7 @@ class GtRange(ctypes.Structure):
 
 def compile_plugin(code: str, name: str, arch: str, options=()) -> bytes:
     """hiprtc-compile a plugin translation unit into a code object for
-    ``arch`` (e.g. ``"gfx950"``).  Raises RuntimeError with the compiler log."""
+    ``arch`` (e.g. ``"gfx950"``).  Raises HipLibraryError with the compiler log."""
     lib = _hip.load_library()
     opts = [o.encode() for o in options]
     arr = (ctypes.c_char_p * max(1, len(opts)))(*opts)
@@ -348,20 +347,6 @@ class _HipOperator(Operator):
         launch_plugin_kernel(kernel, queue.handle(), self._global_shape(output, argument),
                              output, argument, *self._kernel_arguments(output, argument, queue))
 
-
-# Compiled plugins by (device, signature).  Invalidation only forgets them: the
-# modules stay loaded until the process ends, since HIP graphs captured earlier
-# may still launch their kernels.
-_PROGRAM_CACHE: dict[tuple[str, str], PluginProgram] = {}
-_RETIRED_PROGRAMS: list[PluginProgram] = []
-
-
-def invalidate_kernel_cache() -> None:
-    """Forget every compiled plugin, so the next use compiles its sources
-    again (opencl.py:457-459)."""
-    _RETIRED_PROGRAMS.extend(_PROGRAM_CACHE.values())
-    _PROGRAM_CACHE.clear()
-
     def apply_to(self, argument, output=None, queue=None, return_event=False, **kwargs):
         queue = self._infer_queue(argument=argument, output=output, queue=queue)
         stream = queue.torch_stream()
@@ -381,3 +366,17 @@ def invalidate_kernel_cache() -> None:
         if return_event:
             return output, [event]
         return output
+
+
+# Compiled plugins by (device, signature).  Invalidation only forgets them: the
+# modules stay loaded until the process ends, since HIP graphs captured earlier
+# may still launch their kernels.
+_PROGRAM_CACHE: dict[tuple[str, str], PluginProgram] = {}
+_RETIRED_PROGRAMS: list[PluginProgram] = []
+
+
+def invalidate_kernel_cache() -> None:
+    """Forget every compiled plugin, so the next use compiles its sources
+    again (opencl.py:457-459)."""
+    _RETIRED_PROGRAMS.extend(_PROGRAM_CACHE.values())
+    _PROGRAM_CACHE.clear()

@@ -118,6 +118,15 @@ def test_launch_rejects_more_than_three_axes():
         launch_plugin_kernel(0, 0, (2, 2, 2, 2))
 
 
+def test_hip_operators_define_the_execution_pipeline():
+    from gratopy_amd.operator.base import Operator
+    from gratopy_amd.operator.hip import _HipOperator, invalidate_kernel_cache as inv
+
+    assert _HipOperator.apply_to is not Operator.apply_to
+    assert Radon.apply_to is not Operator.apply_to
+    assert not [c for c in inv.__code__.co_consts if hasattr(c, "co_name")]  # no nested defs
+
+
 def test_invalidate_kernel_cache_empties_the_cache():
     _PROGRAM_CACHE[("cuda:0", "dummy")] = object()
     invalidate_kernel_cache()
