@@ -323,6 +323,10 @@ static void launch_back(const gt_batch_plan *p, hipStream_t st, const Job<real> 
                      p->tiles[kStage]);
   const dim3 grid(p->n_tiles[kBackward]);
   switch (p->back_rows) {
+    case 8:
+      hipLaunchKernelGGL((batch_back_kernel<real, 8>), grid, dim3(256), 0, st, jobs,
+                         p->tiles[kBackward]);
+      break;
     case 4:
       hipLaunchKernelGGL((batch_back_kernel<real, 4>), grid, dim3(256), 0, st, jobs,
                          p->tiles[kBackward]);

@@ -764,4 +764,100 @@ __device__ __forceinline__ void band_back_reduce_entry(const Job<real> &J,
   store_back(J, x, y, z, acc);
 }
 
+// ---------------------------------------------------------------------------
+// LDS-staged backprojection.  A block's pixel tile reaches, for each angle,
+// only a short window of the padded sinogram row; the block copies the
+// windows of a chunk of angles into LDS with coalesced loads and reads both
+// taps of every pixel from there, so the per-pixel gathers leave the vector
+// memory pipeline (whose address unit limits the unstaged kernel).
+// ---------------------------------------------------------------------------
+constexpr int kStageWidth = 320;  // LDS elements per staged window
+
+// Window of one angle for one tile: padded-sinogram indices [w0, w0 + n),
+// zero outside [0, Ns + 4).  Coordinates are relative to w0: radon
+// q = c0 x + c1 y + k0 (k0 = c2 - w0); fan t = c0 x + c1 y + k0,
+// ss = c3 x + c4 y + k1, p = ss / t + k2 (k2 = mid_det + 2 - w0).  Every
+// pixel of the tile has 1 <= p < n - 2 (to rounding), so the taps floor(p),
+// floor(p) + 1 need no clamp: where the unstaged kernel clamps p into
+// [0.5, Ns + 2.5], both windows hold zeros.  n = 0 marks an angle read from
+// memory instead (window wider than kStageWidth, or a fan tile reaching
+// t <= 0); it keeps unshifted constants and the clamp [lo, hi].
+template <typename real>
+struct StageWindow {
+  real k0, k1, k2, lo, hi;
+  int w0, n;
+};
+
+// The detector coordinate is a linear (radon) or linear-fractional (fan, with
+// t > 0 on the tile) function of the pixel, so its extremes over the tile
+// [xa, xb] x [ya, yb] (the pixels its lanes evaluate) lie at the corners.  One
+// index of margin on each side covers the rounding of the per-pixel evaluation.
+template <typename real, bool FAN>
+__device__ __forceinline__ StageWindow<real> stage_window(const Job<real> &J, int a, int xa, int xb,
+                                                          int ya, int yb) {
+  const AngleRec<real> &c = J.tab[a];
+  const double mid = FAN ? (double)J.geo[5] + 2.0 : 0.0;
+  double pmin = 0, pmax = 0;
+  bool ok = true;
+  for (int k = 0; k < 4; ++k) {
+    const double x = (k & 1) ? xb : xa, y = (k & 2) ? yb : ya;
+    double p;
+    if (FAN) {
+      const double t = (double)c.c[0] * x + (double)c.c[1] * y + (double)c.c[2];
+      const double ss = (double)c.c[3] * x + (double)c.c[4] * y + (double)c.c[5];
+      ok = ok && t > 0;
+      p = ss / t + mid;
+    } else {
+      p = (double)c.c[0] * x + (double)c.c[1] * y + (double)c.c[2];
+    }
+    pmin = k == 0 ? p : fmin(pmin, p);
+    pmax = k == 0 ? p : fmax(pmax, p);
+  }
+  StageWindow<real> W;
+  const double span = pmax - pmin;
+  ok = ok && span == span && span < kStageWidth - 4 && fabs(pmin) < 1e9;
+  const int w0 = ok ? (int)floor(pmin) - 1 : 0;
+  W.w0 = w0;
+  W.n = ok ? (int)floor(pmax) + 3 - w0 : 0;
+  W.k0 = FAN ? c.c[2] : (real)((double)c.c[2] - w0);
+  W.k1 = FAN ? c.c[5] : (real)0;
+  W.k2 = (real)(mid - w0);
+  W.lo = (real)0.5;
+  W.hi = (real)J.Ns + (real)2.5;
+  return W;
+}
+
+// Adds one angle's contribution to the RY pixels (x, y0 + r) of this lane.
+// STAGED: `row` is the angle's LDS window (no clamp); otherwise the padded
+// sinogram row in memory (clamped as in the unstaged kernel).
+template <typename real, bool FAN, int RY, bool STAGED>
+__device__ __forceinline__ void staged_angle(const AngleRec<real> &c, const StageWindow<real> &W,
+                                             const real *row, real xr, real yr, real *acc) {
+  using A = Arith<real>;
+  real t = FAN ? fma(c.c[0], xr, fma(c.c[1], yr, W.k0)) : (real)1;
+  real q = FAN ? fma(c.c[3], xr, fma(c.c[4], yr, W.k1)) : fma(c.c[0], xr, fma(c.c[1], yr, W.k0));
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    real p, w = 1;
+    if (FAN) {
+      w = A::rcp(t);
+      p = fma(q, w, W.k2);
+    } else {
+      p = q;
+    }
+    if (!STAGED) p = A::clamp(p, W.lo, W.hi);
+    const real f = A::fract(p);
+    const real *tap = row + (int)p;
+    const real v0 = tap[0], v1 = tap[1];
+    const real v = v0 + f * (v1 - v0);
+    acc[r] = FAN ? acc[r] + v * w : acc[r] + v;
+    if (FAN) {
+      t += c.c[1];
+      q += c.c[4];
+    } else {
+      q += c.c[1];
+    }
+  }
+}
+
 }  // namespace gt

@@ -40,8 +40,8 @@ extern "C" int gt_tune_set(int key, int value) {
       t.fwd_group = value;
       return GT_OK;
     case GT_TUNE_BACK_ROWS:
-      if (value != 1 && value != 2 && value != 4)
-        return gt::fail(GT_EINVAL, "backprojection rows per lane must be 1, 2 or 4");
+      if (value != 1 && value != 2 && value != 4 && value != 8)
+        return gt::fail(GT_EINVAL, "backprojection rows per lane must be 1, 2, 4 or 8");
       t.back_rows = value;
       return GT_OK;
     case GT_TUNE_FWD_BANDS:
@@ -53,6 +53,16 @@ extern "C" int gt_tune_set(int key, int value) {
       if (value != 1 && value != 2 && value != 4 && value != 8)
         return gt::fail(GT_EINVAL, "backprojection angle bands must be 1, 2, 4 or 8");
       t.back_bands = value;
+      return GT_OK;
+    case GT_TUNE_BACK_STAGE:
+      if (value != 0 && value != 8 && value != 16)
+        return gt::fail(GT_EINVAL, "backprojection staging chunk must be 0, 8 or 16 angles");
+      t.back_stage = value;
+      return GT_OK;
+    case GT_TUNE_BACK_TILE:
+      if (value != 16 && value != 32)
+        return gt::fail(GT_EINVAL, "staged backprojection tile width must be 16 or 32");
+      t.back_tile = value;
       return GT_OK;
     default:
       return gt::fail(GT_EINVAL, "unknown tuning key");
@@ -70,6 +80,10 @@ extern "C" int gt_tune_get(int key) {
       return t.fwd_bands;
     case GT_TUNE_BACK_BANDS:
       return t.back_bands;
+    case GT_TUNE_BACK_STAGE:
+      return t.back_stage;
+    case GT_TUNE_BACK_TILE:
+      return t.back_tile;
     default:
       return -1;
   }

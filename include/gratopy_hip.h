@@ -45,7 +45,7 @@ int gt_version(void);
  * driver (gratopy/gratopy.py:304-313).  Results do not depend on them beyond
  * rounding.
  *   GT_TUNE_FWD_GROUP: detector bins per lane in the forward projection (1, 2, 4, 8)
- *   GT_TUNE_BACK_ROWS: pixels per lane in the backprojection (1, 2, 4)
+ *   GT_TUNE_BACK_ROWS: pixels per lane in the backprojection (1, 2, 4, 8)
  *   GT_TUNE_FWD_BANDS: row bands of the forward strip walk (1, 2, 4, 8; applies when
  *                      GT_TUNE_FWD_GROUP > 1).  With B > 1 bands, consecutive thread
  *                      blocks take different bands (so each XCD's L2 holds one band of
@@ -53,8 +53,22 @@ int gt_version(void);
  *                      for the per-band partial sums.
  *   GT_TUNE_BACK_BANDS: angle bands of the backprojection (1, 2, 4, 8), on the same
  *                      pattern (each XCD's L2 holds the padded sinogram rows of one band;
- *                      the workspace grows by B * Nx * Ny * Nz elements). */
-enum { GT_TUNE_FWD_GROUP = 0, GT_TUNE_BACK_ROWS = 1, GT_TUNE_FWD_BANDS = 2, GT_TUNE_BACK_BANDS = 3 };
+ *                      the workspace grows by B * Nx * Ny * Nz elements).
+ *   GT_TUNE_BACK_STAGE: angles per chunk of the LDS-staged backprojection (0 = off, 8,
+ *                      16; applies with one angle band): each block copies the sinogram
+ *                      window its pixel tile reaches, per angle, into LDS and reads the
+ *                      interpolation taps from there.
+ *   GT_TUNE_BACK_TILE: lanes along x of a staged backprojection tile (16 or 32; the
+ *                      tile is that many pixels wide and 256 / width * rows-per-lane
+ *                      pixels tall). */
+enum {
+  GT_TUNE_FWD_GROUP = 0,
+  GT_TUNE_BACK_ROWS = 1,
+  GT_TUNE_FWD_BANDS = 2,
+  GT_TUNE_BACK_BANDS = 3,
+  GT_TUNE_BACK_STAGE = 4,
+  GT_TUNE_BACK_TILE = 5
+};
 int gt_tune_set(int key, int value);
 int gt_tune_get(int key);
 

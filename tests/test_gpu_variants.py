@@ -1,6 +1,7 @@
 """Every launch variant the autotuner can pick (forward bin groups K = 1, 2,
-4, 8, with or without 8 row bands; backprojection rows per lane RY = 1, 2, 4,
-with or without 8 angle bands)
+4, 8, with or without 8 row bands; backprojection rows per lane RY = 1, 2, 4, 8,
+with or without 8 angle bands, or with sinogram windows staged in LDS for
+chunks of 8 or 16 angles in tiles 16 or 32 pixels wide)
 meets the parity bar against
 the oracle: normwise 1e-5 in float32, 1e-12 in float64, on regular, shifted,
 ragged and degenerate geometries (including fan bins so wide that a bin
@@ -35,7 +36,7 @@ def restore_variant():
     before = autotune.current_variant()
     yield
     autotune.set_variant(before["fwd_group"], before["back_rows"], before["fwd_bands"],
-                         before["back_bands"])
+                         before["back_bands"], before["back_stage"], before["back_tile"])
 
 
 @pytest.mark.parametrize("K,bands", autotune.FWD_VARIANTS)
@@ -53,10 +54,10 @@ def test_forward_variants(K, bands, restore_variant):
             assert np.abs(got - ref).max() <= TOL[dtype] * scale, (geometry, shape, K, bands, dtype)
 
 
-@pytest.mark.parametrize("RY,bands", autotune.BACK_VARIANTS)
-def test_backward_variants(RY, bands, restore_variant):
-    print(f"variant RY={RY} bands={bands}", flush=True)
-    autotune.set_variant(back_rows=RY, back_bands=bands)
+@pytest.mark.parametrize("RY,bands,stage,tile", autotune.BACK_VARIANTS)
+def test_backward_variants(RY, bands, stage, tile, restore_variant):
+    print(f"variant RY={RY} bands={bands} stage={stage} tile={tile}", flush=True)
+    autotune.set_variant(back_rows=RY, back_bands=bands, back_stage=stage, back_tile=tile)
     for geometry, shape, na, nd, kw in CASES:
         g = gratopy.RADON if geometry == "radon" else gratopy.FANBEAM
         ps = gratopy.ProjectionSettings("cuda:0", g, shape, na, nd, **kw)
@@ -65,7 +66,8 @@ def test_backward_variants(RY, bands, restore_variant):
             ref = _oracle(ps, sino, dtype, True)
             got = _hip(ps, sino, dtype, True, in_order="C", out_order="C")
             scale = max(np.abs(ref).max(), 1e-3)
-            assert np.abs(got - ref).max() <= TOL[dtype] * scale, (geometry, shape, RY, bands, dtype)
+            assert np.abs(got - ref).max() <= TOL[dtype] * scale, (geometry, shape, RY, bands, stage,
+                                                                   tile, dtype)
 
 
 _LARGE = {}
@@ -117,7 +119,8 @@ def test_autotuner_picks_a_variant_and_caches_it(tmp_path, monkeypatch, restore_
     best = autotune.tune(ps, img.clone(), sino.clone(), repeats=2)
     assert best["fwd_group"] in autotune.FWD_GROUPS and best["back_rows"] in autotune.BACK_ROWS
     assert autotune.current_variant() == {"fwd_group": best["fwd_group"], "back_rows": best["back_rows"],
-                                          "fwd_bands": best["fwd_bands"], "back_bands": best["back_bands"]}
+                                          "fwd_bands": best["fwd_bands"], "back_bands": best["back_bands"],
+                                          "back_stage": best["back_stage"], "back_tile": best["back_tile"]}
     autotune.set_variant(1, 1, 1, 1)
     assert autotune.apply(ps, F32, 1)
     assert autotune.current_variant()["fwd_group"] == best["fwd_group"]
