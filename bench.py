@@ -49,6 +49,8 @@ def parse():
                    help="angles per LDS staging chunk of the backprojection (0 = unstaged, 8, 16)")
     p.add_argument("--back-tile", type=int, default=None,
                    help="tile width (16 or 32 pixels) of the staged backprojection")
+    p.add_argument("--fwd-chunk", type=int, default=None,
+                   help="forward strip reads: 1 pixel or 4 (16 bytes) per load")
     p.add_argument("--no-batched", action="store_true",
                    help="skip the batched-vs-plain solver measurements")
     p.add_argument("--tv-iterations", type=int, default=6,
@@ -268,6 +270,8 @@ def main():
         _hip.check(lib.gt_tune_set(4, args.back_stage), "gt_tune_set")
     if args.back_tile is not None:
         _hip.check(lib.gt_tune_set(5, args.back_tile), "gt_tune_set")
+    if args.fwd_chunk is not None:
+        _hip.check(lib.gt_tune_set(6, args.fwd_chunk), "gt_tune_set")
     Nz, N, Na, Ns, RE, R, iw, dw, n_ell = WORKLOADS[args.workload]
     ps_kwargs = dict(R=R, RE=RE, image_width=iw, detector_width=dw)
     device = "cuda:0"
@@ -281,7 +285,7 @@ def main():
     tuning_source = "fixed"
     tuned = None
     if all(v is None for v in (args.fwd_group, args.back_rows, args.fwd_bands, args.back_bands,
-                               args.back_stage, args.back_tile)):
+                               args.back_stage, args.back_tile, args.fwd_chunk)):
         from gratopy_amd import autotune
 
         back.copy_(img)
@@ -349,6 +353,7 @@ def main():
         "tuning": {"fwd_group": lib.gt_tune_get(0), "fwd_bands": lib.gt_tune_get(2),
                    "back_rows": lib.gt_tune_get(1), "back_bands": lib.gt_tune_get(3),
                    "back_stage": lib.gt_tune_get(4), "back_tile": lib.gt_tune_get(5),
+                   "fwd_chunk": lib.gt_tune_get(6),
                    "source": tuning_source,
                    "trial_ms": None if tuned is None else {"fwd": tuned["fwd_ms"], "back": tuned["back_ms"]}},
         "fwd_ms": round(fwd_med * 1e3, 3),

@@ -1,12 +1,14 @@
 """Launch-shape autotuner for the projection kernels.
 
-The forward kernels assign K adjacent detector bins to a lane (K in 1, 2,
-4, 8; K = 1 is the per-ray kernel), optionally split the strip walk into 8
-row bands (one per XCD, partial sums added in band order), and the
-backprojection assigns RY pixels along y to a lane (RY in 1, 2, 4, 8),
-optionally over 8 angle bands on the same pattern, or stages each block's
-sinogram windows in LDS for chunks of 8 or 16 angles (tiles 16 or 32 pixels
-wide).
+Knobs (the C ABI's ``GT_TUNE_*`` keys): the forward kernels assign K
+adjacent detector bins to a lane (``fwd_group`` K in 1, 2, 4, 8, 16; K = 1 is the
+per-ray kernel), optionally split the strip walk into 8 row bands
+(``fwd_bands``; one band per XCD, partial sums added in band order) and read
+strip rows 16 bytes per load (``fwd_chunk`` = 4).  The backprojection assigns
+RY pixels along y to a lane (``back_rows`` in 1, 2, 4, 8) and either sums all
+angles per tile, splits them into 8 angle bands (``back_bands``), or stages
+each tile's sinogram windows in LDS for chunks of 8 or 16 angles
+(``back_stage``) in tiles 16 or 32 pixels wide (``back_tile``).
 Batched launches use K and RY only.  :func:`tune` times every variant on the
 caller's own geometry and array shapes with HIP events, keeps the fastest,
 and remembers it per shape key in a JSON cache
@@ -27,18 +29,29 @@ import numpy as np
 from . import _device as dev
 from . import _hip
 
-FWD_GROUPS = (1, 2, 4, 8)
+# knob name -> GT_TUNE_* key of the C ABI
+KNOBS = {"fwd_group": 0, "back_rows": 1, "fwd_bands": 2, "back_bands": 3, "back_stage": 4,
+         "back_tile": 5, "fwd_chunk": 6}
+# the library's values at load time
+DEFAULTS = {"fwd_group": 4, "back_rows": 1, "fwd_bands": 1, "back_bands": 1, "back_stage": 0,
+            "back_tile": 16, "fwd_chunk": 1}
+
+FWD_GROUPS = (1, 2, 4, 8, 16)
 BACK_ROWS = (1, 2, 4, 8)
 FWD_BANDS = (1, 8)
+FWD_CHUNKS = (1, 4)
 BACK_STAGES = (0, 8, 16)
 BACK_TILES = (16, 32)
-# forward variants: (bins per lane, row bands); bands need bin groups
-FWD_VARIANTS = tuple((k, 1) for k in FWD_GROUPS) + tuple((k, 8) for k in FWD_GROUPS if k > 1)
-# backprojection variants: (rows per lane, angle bands, LDS staging chunk, staged tile width)
-BACK_VARIANTS = (tuple((r, 1, 0, 16) for r in BACK_ROWS) + tuple((r, 8, 0, 16) for r in BACK_ROWS)
-                 + tuple((r, 1, s, t) for s in BACK_STAGES if s for t in BACK_TILES
-                         for r in BACK_ROWS))
-KEY_FWD, KEY_BACK, KEY_BANDS, KEY_BACK_BANDS, KEY_BACK_STAGE, KEY_BACK_TILE = 0, 1, 2, 3, 4, 5
+# forward variants; bands and chunked reads need bin groups
+FWD_VARIANTS = ({"fwd_group": 1, "fwd_bands": 1, "fwd_chunk": 1},) + tuple(
+    {"fwd_group": k, "fwd_bands": b, "fwd_chunk": c}
+    for k in FWD_GROUPS if k > 1 for b in FWD_BANDS for c in FWD_CHUNKS)
+# backprojection variants: unstaged (with or without angle bands) and staged
+BACK_VARIANTS = tuple(
+    {"back_rows": r, "back_bands": b, "back_stage": 0, "back_tile": 16}
+    for b in (1, 8) for r in BACK_ROWS) + tuple(
+    {"back_rows": r, "back_bands": 1, "back_stage": s, "back_tile": t}
+    for s in BACK_STAGES if s for t in BACK_TILES for r in BACK_ROWS)
 
 _lock = threading.Lock()
 
@@ -77,39 +90,36 @@ def _store(table: dict) -> None:
         pass  # a read-only home only loses the cache
 
 
-def set_variant(fwd_group: int | None = None, back_rows: int | None = None,
-                fwd_bands: int | None = None, back_bands: int | None = None,
-                back_stage: int | None = None, back_tile: int | None = None) -> None:
+def set_variant(**knobs) -> None:
+    """Set launch-shape knobs (names of :data:`KNOBS`; None leaves a knob)."""
     lib = _hip.load_library()
-    if fwd_group is not None:
-        _hip.check(lib.gt_tune_set(KEY_FWD, int(fwd_group)), "gt_tune_set")
-    if back_rows is not None:
-        _hip.check(lib.gt_tune_set(KEY_BACK, int(back_rows)), "gt_tune_set")
-    if fwd_bands is not None:
-        _hip.check(lib.gt_tune_set(KEY_BANDS, int(fwd_bands)), "gt_tune_set")
-    if back_bands is not None:
-        _hip.check(lib.gt_tune_set(KEY_BACK_BANDS, int(back_bands)), "gt_tune_set")
-    if back_stage is not None:
-        _hip.check(lib.gt_tune_set(KEY_BACK_STAGE, int(back_stage)), "gt_tune_set")
-    if back_tile is not None:
-        _hip.check(lib.gt_tune_set(KEY_BACK_TILE, int(back_tile)), "gt_tune_set")
+    for name, value in knobs.items():
+        if name not in KNOBS:
+            raise ValueError(f"unknown tuning knob {name!r}")
+        if value is not None:
+            _hip.check(lib.gt_tune_set(KNOBS[name], int(value)), "gt_tune_set")
 
 
 def current_variant() -> dict:
     lib = _hip.load_library()
-    return {"fwd_group": lib.gt_tune_get(KEY_FWD), "back_rows": lib.gt_tune_get(KEY_BACK),
-            "fwd_bands": lib.gt_tune_get(KEY_BANDS), "back_bands": lib.gt_tune_get(KEY_BACK_BANDS),
-            "back_stage": lib.gt_tune_get(KEY_BACK_STAGE), "back_tile": lib.gt_tune_get(KEY_BACK_TILE)}
+    return {name: lib.gt_tune_get(key) for name, key in KNOBS.items()}
 
 
-def _fwd_key(k: int, b: int) -> str:
-    return f"K{k}" if b == 1 else f"K{k}xB{b}"
-
-
-def _back_key(r: int, b: int, s: int = 0, t: int = 16) -> str:
-    if s:
-        return f"RY{r}xS{s}T{t}"
-    return f"RY{r}" if b == 1 else f"RY{r}xB{b}"
+def variant_key(v: dict) -> str:
+    """Short name of a variant, e.g. ``K4xB8xC4`` or ``RY8xS16T32``."""
+    if "fwd_group" in v:
+        key = f"K{v['fwd_group']}"
+        if v.get("fwd_bands", 1) > 1:
+            key += f"xB{v['fwd_bands']}"
+        if v.get("fwd_chunk", 1) > 1:
+            key += f"xC{v['fwd_chunk']}"
+        return key
+    key = f"RY{v['back_rows']}"
+    if v.get("back_stage", 0):
+        return key + f"xS{v['back_stage']}T{v.get('back_tile', 16)}"
+    if v.get("back_bands", 1) > 1:
+        key += f"xB{v['back_bands']}"
+    return key
 
 
 def _time(fn, repeats: int) -> float:
@@ -131,48 +141,42 @@ def tune(projectionsetting, img, sino, repeats: int = 3, store: bool = True) -> 
     """Time all variants on (img, sino) and install the fastest.
 
     ``img`` and ``sino`` are device arrays of the shapes and orders the caller
-    will use; their contents are overwritten.
+    will use; their contents are overwritten.  Returns the chosen knobs plus
+    the trial times (ms) per variant under ``fwd_ms`` and ``back_ms``.
     """
     from .gratopy import backprojection, forwardprojection
 
     ps = projectionsetting
-    fwd_times = {}
-    for k, b in FWD_VARIANTS:
-        set_variant(fwd_group=k, fwd_bands=b)
-        fwd_times[(k, b)] = _time(lambda: forwardprojection(img, ps, sino=sino), repeats)
-    back_times = {}
-    for r, b, s, t in BACK_VARIANTS:
-        set_variant(back_rows=r, back_bands=b, back_stage=s, back_tile=t)
-        back_times[(r, b, s, t)] = _time(lambda: backprojection(sino, ps, img=img), repeats)
-    k, b = min(fwd_times, key=fwd_times.get)
-    r, rb, rs, rt = min(back_times, key=back_times.get)
-    best = {"fwd_group": k, "fwd_bands": b, "back_rows": r, "back_bands": rb, "back_stage": rs,
-            "back_tile": rt,
-            "fwd_ms": {_fwd_key(*v): t for v, t in fwd_times.items()},
-            "back_ms": {_back_key(*v): t for v, t in back_times.items()}}
-    set_variant(best["fwd_group"], best["back_rows"], best["fwd_bands"], best["back_bands"],
-                best["back_stage"], best["back_tile"])
+    fwd_times, back_times = {}, {}
+    for v in FWD_VARIANTS:
+        set_variant(**v)
+        fwd_times[variant_key(v)] = (_time(lambda: forwardprojection(img, ps, sino=sino), repeats), v)
+    for v in BACK_VARIANTS:
+        set_variant(**v)
+        back_times[variant_key(v)] = (_time(lambda: backprojection(sino, ps, img=img), repeats), v)
+    best_fwd = min(fwd_times.values(), key=lambda tv: tv[0])[1]
+    best_back = min(back_times.values(), key=lambda tv: tv[0])[1]
+    knobs = {**best_fwd, **best_back}
+    set_variant(**knobs)
     if store:
         nz = int(img.shape[2]) if img.dim() > 2 else 1
         with _lock:
             table = _load()
-            table[shape_key(ps, dev.to_np_dtype(img.dtype), nz)] = {
-                "fwd_group": best["fwd_group"], "fwd_bands": best["fwd_bands"],
-                "back_rows": best["back_rows"], "back_bands": best["back_bands"],
-                "back_stage": best["back_stage"], "back_tile": best["back_tile"]}
+            table[shape_key(ps, dev.to_np_dtype(img.dtype), nz)] = knobs
             _store(table)
-    return best
+    return {**knobs, "fwd_ms": {k: t for k, (t, _) in fwd_times.items()},
+            "back_ms": {k: t for k, (t, _) in back_times.items()}}
 
 
 def tune_batch(settings, imgs, sinos, repeats: int = 2) -> dict:
     """Time all variants on a batch (the tile lists depend on the variant, so
     one plan is built per variant) and install the fastest.  ``imgs`` and
-    ``sinos`` are overwritten.  Batched launches use no bands, so the bands
-    are switched off: single-slice calls then run the same kernels as the
-    batch."""
+    ``sinos`` are overwritten.  Batched launches use neither bands, chunked
+    reads nor LDS staging, so those are switched off: single-slice calls then
+    run the same kernels as the batch."""
     from .batch import BatchPlan
 
-    set_variant(fwd_bands=1, back_bands=1, back_stage=0)
+    set_variant(fwd_bands=1, back_bands=1, back_stage=0, fwd_chunk=1)
 
     def timed(op):
         plan = BatchPlan(settings, imgs, sinos)
@@ -190,15 +194,15 @@ def tune_batch(settings, imgs, sinos, repeats: int = 2) -> dict:
         back_times[r] = timed(_hip.BATCH_BACKWARD)
     best = {"fwd_group": min(fwd_times, key=fwd_times.get), "back_rows": min(back_times, key=back_times.get),
             "fwd_ms": fwd_times, "back_ms": back_times}
-    set_variant(best["fwd_group"], best["back_rows"])
+    set_variant(fwd_group=best["fwd_group"], back_rows=best["back_rows"])
     return best
 
 
 def apply(projectionsetting, dtype, nz: int = 1) -> bool:
-    """Install the cached variant for this shape; False when none is cached."""
+    """Install the cached variant for this shape (knobs a cache entry lacks
+    take their defaults); False when none is cached."""
     entry = _load().get(shape_key(projectionsetting, dtype, nz))
     if not entry:
         return False
-    set_variant(entry.get("fwd_group"), entry.get("back_rows"), entry.get("fwd_bands", 1),
-                entry.get("back_bands", 1), entry.get("back_stage", 0), entry.get("back_tile", 16))
+    set_variant(**{name: entry.get(name, DEFAULTS[name]) for name in KNOBS})
     return True

@@ -222,6 +222,9 @@ static int plan_create(gt_batch_plan **out, int dtype, int n, const gt_batch_ite
     J.sino = static_cast<real *>(it.sino);
     J.so = order_strides(GT_ORDER_F, it.Ns, it.Na, 1);
     J.imgY = reinterpret_cast<const real *>(base + Y_at[i]);
+    J.rowX = it.Nx;
+    J.rowY = it.Ny;
+    J.sliceX = J.sliceY = (int64_t)it.Nx * it.Ny;
     J.io = order_strides(GT_ORDER_F, it.Nx, it.Ny, 1);
     J.S = reinterpret_cast<const real *>(base + S_at[i]);
     J.tab = reinterpret_cast<const AngleRec<real> *>(base + tab_at[i]);
@@ -303,6 +306,9 @@ static void launch_fwd(const gt_batch_plan *p, hipStream_t st, const Job<real> *
                        jobs, p->tiles[kRelayout]);
   const dim3 grid(p->n_tiles[kForward]), block(64, 4);
   switch (p->fwd_group) {
+    case 16:
+      hipLaunchKernelGGL((batch_fwd_kernel<real, 16>), grid, block, 0, st, jobs, p->tiles[kForward]);
+      break;
     case 8:
       hipLaunchKernelGGL((batch_fwd_kernel<real, 8>), grid, block, 0, st, jobs, p->tiles[kForward]);
       break;

@@ -29,8 +29,10 @@ struct alignas(sizeof(real) * 8) AngleRec {
   real c[8];
 };
 
-// Everything one projection needs.  Forward jobs use sino/so/imgX/imgY,
-// backward jobs img/io/S/tab.  Epilogues (data shares the sinogram's strides):
+// Everything one projection needs.  Forward jobs use sino/so/imgX/imgY (an
+// x-fastest and a y-fastest copy of the image: row strides rowX/rowY, slice
+// strides sliceX/sliceY; rows may be padded with zeros), backward jobs
+// img/io/S/tab.  Epilogues (data shares the sinogram's strides):
 //   mode 0: store the projection;
 //   mode 1: forward stores value - data[k] (Landweber residual), backward
 //           stores img[k] + scale * value (Landweber update);
@@ -49,6 +51,7 @@ struct Job {
   real *sino;
   Strides3 so;
   const real *imgX, *imgY;
+  int64_t rowX, rowY, sliceX, sliceY;
   real *img;
   Strides3 io;
   const real *S;
@@ -235,6 +238,20 @@ __device__ inline void fan_rows(double qx, double qy, double dpx, double dmx, do
   clip_rows(lo + midy + qy, hi + midy + qy, Nyy, ylo, yhi);
 }
 
+// CH consecutive elements of an image row in one load (CH * sizeof(real) <= 16
+// bytes; element alignment suffices).
+template <typename real, int CH>
+struct RowChunk {
+  real v[CH];
+};
+
+template <typename real, int CH>
+__device__ __forceinline__ RowChunk<real, CH> load_chunk(const real *p) {
+  RowChunk<real, CH> c;
+  __builtin_memcpy(c.v, p, sizeof(c.v));
+  return c;
+}
+
 // ---------------------------------------------------------------------------
 // Fan ray geometry of a strip bounded by the rays through detector positions
 // s_lo and s_hi (fanbeam.cl:100-184 with s-1, s+1 replaced by s_lo, s_hi).
@@ -281,10 +298,9 @@ __device__ __forceinline__ FanStrip<real> fan_strip(const Job<real> &J, int a, i
   F.RE *= (real)1 / R;
   F.Nxx = J.Nx;
   F.Nyy = J.Ny;
-  const int64_t slice = (int64_t)z * J.Nx * J.Ny;
   if (fabs(dpx) < fabs(dpy) && fabs(dmx) < fabs(dmy)) {
-    F.base = J.imgX + slice;
-    F.row_stride = J.Nx;
+    F.base = J.imgX + z * J.sliceX;
+    F.row_stride = J.rowX;
   } else {
     real t;
     t = ddx; ddx = ddy; ddy = t;
@@ -295,8 +311,8 @@ __device__ __forceinline__ FanStrip<real> fan_strip(const Job<real> &J, int a, i
     t = F.midx; F.midx = F.midy; F.midy = t;
     F.Nyy = J.Nx;
     F.Nxx = J.Ny;
-    F.base = J.imgY + slice;
-    F.row_stride = J.Ny;
+    F.base = J.imgY + z * J.sliceY;
+    F.row_stride = J.rowY;
   }
   F.regular = dpy * dmy > (real)0;
   dpx /= dpy;
@@ -325,18 +341,17 @@ __device__ __forceinline__ void radon_fwd_ray(const Job<real> &J, int s, int a, 
   int Nxx = J.Nx, Nyy = J.Ny;
   const real *base;
   int64_t row_stride;
-  const int64_t slice = (int64_t)z * J.Nx * J.Ny;
   if (o[5] != (real)0) {
     const real t = X;
     X = Y;
     Y = t;
     Nxx = J.Ny;
     Nyy = J.Nx;
-    base = J.imgY + slice;
-    row_stride = J.Ny;
+    base = J.imgY + z * J.sliceY;
+    row_stride = J.rowY;
   } else {
-    base = J.imgX + slice;
-    row_stride = J.Nx;
+    base = J.imgX + z * J.sliceX;
+    row_stride = J.rowX;
   }
   int ylo, yhi;
   radon_rows((double)X, (double)Y, (double)off - (double)s, 1.0, Nxx, Nyy, ylo, yhi);
@@ -433,7 +448,12 @@ __device__ __forceinline__ void band_rows(int Nyy, int index, int count, int &yl
 }
 
 // acc: this lane's slot 0 in an LDS array of Slots<K>::kCount rows of 256.
-template <typename real, int K, bool BANDED = false>
+// CH > 1: the strip row is read CH pixels per load.  The last load of a row
+// may reach up to CH - 1 pixels past the strip: inside the row those lie
+// outside the union strip, so their hat weights land in the discarded slots
+// (0, 1, K + 2, K + 3); past the end of the row they are the copy's zero
+// padding (the single-problem launches pad every row by 4 elements).
+template <typename real, int K, bool BANDED = false, int CH = 1>
 __device__ __forceinline__ void fan_fwd_group(const Job<real> &J, int s0, int a, int z, real *acc,
                                               Band<real> B = {}) {
   using A = Arith<real>;
@@ -459,11 +479,19 @@ __device__ __forceinline__ void fan_fwd_group(const Job<real> &J, int s0, int a,
   // The detector position relative to bin s0 - 2 (the slot index) is
   // p = q/t with q = ss + (mid_det - s0 + 2)*t.  q is carried instead of ss
   // (|ss/t| is up to Ns/2, q/t lies in [1, K+2]), so single precision does
-  // not lose the bin fraction to cancellation; each row starts from a double
-  // evaluation.  p is positive, so truncation is floor.
+  // not lose the bin fraction to cancellation; each row starts from double
+  // values: t and q at pixel column 0 of row ylo (t00, q00) plus the row and
+  // column terms, one FMA each (no running sum: q at column 0 is up to ~Ns in
+  // size, so summing row steps would drift).  p is positive, so truncation
+  // is floor.
   const double c0 = (double)J.geo[5] - (double)s0 + 2.0;
-  const real dq = (real)((double)F.dx + c0 * (double)F.ddx);
+  const double tdx = F.ddx, tdy = F.ddy;
+  const double qdx = (double)F.dx + c0 * tdx, qdy = (double)F.dy + c0 * tdy;
+  const real dq = (real)qdx;
   const real lo = (real)0.5, hi = (real)K + (real)2.5;
+  const double ud0 = (double)ylo - (double)F.midy;
+  const double t00 = tdy * ud0 + (double)F.RE - tdx * (double)F.midx;
+  const double q00 = qdy * ud0 + c0 * (double)F.RE - qdx * (double)F.midx;
   for (int y = ylo; y <= yhi; ++y) {
     const real u = (real)y - F.midy;
     const real xhigh = F.qx + F.dpx * (u - F.qy);
@@ -471,21 +499,26 @@ __device__ __forceinline__ void fan_fwd_group(const Job<real> &J, int s0, int a,
     const int xh = (int)floor(fmax(fmin(xcap, xhigh) + F.midx, (real)-1));
     const int xl = (int)ceil(fmin(fmax(-F.midx, xlow) + F.midx, (real)F.Nxx));
     if (xl > xh) continue;
-    const double xr = (double)xl - (double)F.midx, ud = (double)y - (double)F.midy;
-    const double t0 = (double)F.ddx * xr + (double)F.ddy * ud + (double)F.RE;
-    real t = (real)t0;
-    real q = (real)((double)F.dx * xr + (double)F.dy * ud + c0 * t0);
+    const double dyr = (double)(y - ylo), xd = (double)xl;
+    real t = (real)fma(tdx, xd, fma(tdy, dyr, t00));
+    real q = (real)fma(qdx, xd, fma(qdy, dyr, q00));
+    // walked by address: the end pointer doubles as the loop bound
     const real *row = F.base + (int64_t)y * F.row_stride;
-    for (int x = xl; x <= xh; ++x) {
-      const real r = A::rcp(t);
-      const real p = A::clamp(q * r, lo, hi);
-      const real f = A::fract(p);
-      const real v = row[x] * r;
-      real *slot = acc + (int)p * 256;
-      slot[0] += v - f * v;
-      slot[256] += f * v;
-      t += F.ddx;
-      q += dq;
+    const real *end = row + xh;
+    for (const real *px = row + xl; px <= end; px += CH) {
+      const RowChunk<real, CH> c = load_chunk<real, CH>(px);
+#pragma unroll
+      for (int i = 0; i < CH; ++i) {
+        const real r = A::rcp(t);
+        const real p = A::clamp(q * r, lo, hi);
+        const real f = A::fract(p);
+        const real v = c.v[i] * r;
+        real *slot = acc + (int)p * 256;
+        slot[0] += v - f * v;
+        slot[256] += f * v;
+        t += F.ddx;
+        q += dq;
+      }
     }
   }
 #pragma unroll
@@ -507,7 +540,8 @@ __device__ __forceinline__ real radon_scale(const Job<real> &J) {
   return (real)delta_x * (real)delta_x / (real)delta_xi;
 }
 
-template <typename real, int K, bool BANDED = false>
+// CH: pixels per load along the strip row, as in fan_fwd_group.
+template <typename real, int K, bool BANDED = false, int CH = 1>
 __device__ __forceinline__ void radon_fwd_group(const Job<real> &J, int s0, int a, int z,
                                                 real *acc, Band<real> B = {}) {
   using A = Arith<real>;
@@ -519,18 +553,17 @@ __device__ __forceinline__ void radon_fwd_group(const Job<real> &J, int s0, int 
   int Nxx = J.Nx, Nyy = J.Ny;
   const real *base;
   int64_t row_stride;
-  const int64_t slice = (int64_t)z * J.Nx * J.Ny;
   if (o[5] != (real)0) {
     const real t = X;
     X = Y;
     Y = t;
     Nxx = J.Ny;
     Nyy = J.Nx;
-    base = J.imgY + slice;
-    row_stride = J.Ny;
+    base = J.imgY + z * J.sliceY;
+    row_stride = J.rowY;
   } else {
-    base = J.imgX + slice;
-    row_stride = J.Nx;
+    base = J.imgX + z * J.sliceX;
+    row_stride = J.rowX;
   }
   // union strip of the bins [s0, s0+K): centre s0 + (K-1)/2, half width (K+1)/2
   int ylo, yhi;
@@ -539,6 +572,7 @@ __device__ __forceinline__ void radon_fwd_group(const Job<real> &J, int s0, int 
   if (BANDED) band_rows(Nyy, B.index, B.count, ylo, yhi);
   const real lo = (real)0.5, hi = (real)K + (real)2.5;
   const real sl = (real)(s0 - 1), sh = (real)(s0 + K);
+  const double p00 = (double)off - (double)s0 + 2.0;  // pixel (0, 0) relative to bin s0 - 2
   for (int y = ylo; y <= yhi; ++y) {
     const real d = (real)y * Y + off;  // detector position of (0, y)
     int xl = (int)((sl - d) * inv);
@@ -554,16 +588,21 @@ __device__ __forceinline__ void radon_fwd_group(const Job<real> &J, int s0, int 
     // detector position of (xl, y) relative to bin s0 - 2 (the slot index),
     // evaluated in double: x*X and y*Y + off are up to Ns in size and nearly
     // cancel; inside the strip it lies in [1, K+2], so truncation is floor
-    real pr = (real)((double)xl * (double)X + (double)y * (double)Y + (double)off - (double)s0 + 2.0);
+    real pr = (real)fma((double)X, (double)xl, fma((double)Y, (double)y, p00));
     const real *row = base + (int64_t)y * row_stride;
-    for (int x = xl; x <= xh; ++x) {
-      const real v = row[x];
-      const real p = A::clamp(pr, lo, hi);
-      const real f = A::fract(p);
-      real *slot = acc + (int)p * 256;
-      slot[0] += v - f * v;
-      slot[256] += f * v;
-      pr += X;
+    const real *end = row + xh;
+    for (const real *px = row + xl; px <= end; px += CH) {
+      const RowChunk<real, CH> c = load_chunk<real, CH>(px);
+#pragma unroll
+      for (int i = 0; i < CH; ++i) {
+        const real v = c.v[i];
+        const real p = A::clamp(pr, lo, hi);
+        const real f = A::fract(p);
+        real *slot = acc + (int)p * 256;
+        slot[0] += v - f * v;
+        slot[256] += f * v;
+        pr += X;
+      }
     }
   }
   const real scale = radon_scale(J);

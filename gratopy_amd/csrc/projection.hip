@@ -32,8 +32,8 @@ static void launch_relayout(hipStream_t st, real *dst, Strides3 ds, const real *
 }
 
 // K = 1: one lane per detector bin (the reference's loops); K > 1: a lane
-// owns bins [K*i, K*i + K).
-template <typename real, bool FAN, int K>
+// owns bins [K*i, K*i + K) and reads its strip rows CH pixels per load.
+template <typename real, bool FAN, int K, int CH = 1>
 __global__ __launch_bounds__(256) void fwd_kernel(Job<real> J) {
   const int a = blockIdx.y * 4 + threadIdx.y;
   const int z = blockIdx.z;
@@ -50,16 +50,16 @@ __global__ __launch_bounds__(256) void fwd_kernel(Job<real> J) {
     if (s0 >= J.Ns || a >= J.Na) return;
     real *acc = slots + threadIdx.y * 64 + threadIdx.x;
     if (FAN)
-      fan_fwd_group<real, K>(J, s0, a, z, acc);
+      fan_fwd_group<real, K, false, CH>(J, s0, a, z, acc);
     else
-      radon_fwd_group<real, K>(J, s0, a, z, acc);
+      radon_fwd_group<real, K, false, CH>(J, s0, a, z, acc);
   }
 }
 
 // Banded form of the bin-group forward: blockIdx.x = band + nbands * group,
 // so consecutive blocks, which the dispatcher hands to the 8 XCDs in turn,
 // walk different row bands and each XCD's L2 serves one band of the image.
-template <typename real, bool FAN, int K>
+template <typename real, bool FAN, int K, int CH = 1>
 __global__ __launch_bounds__(256) void fwd_band_kernel(Job<real> J, real *part, int nbands) {
   __shared__ real slots[Slots<K>::kCount * 256];
   const int band = blockIdx.x % nbands;
@@ -70,9 +70,9 @@ __global__ __launch_bounds__(256) void fwd_band_kernel(Job<real> J, real *part, 
   real *acc = slots + threadIdx.y * 64 + threadIdx.x;
   const Band<real> B{band, nbands, part};
   if (FAN)
-    fan_fwd_group<real, K, true>(J, s0, a, z, acc, B);
+    fan_fwd_group<real, K, true, CH>(J, s0, a, z, acc, B);
   else
-    radon_fwd_group<real, K, true>(J, s0, a, z, acc, B);
+    radon_fwd_group<real, K, true, CH>(J, s0, a, z, acc, B);
 }
 
 template <typename real>
@@ -354,13 +354,25 @@ static size_t elem_size(int dtype) { return dtype == GT_FLOAT64 ? 8 : 4; }
 // row bands of the forward for the current tuning (bands need bin groups)
 static int fwd_bands() { return tuning().fwd_group > 1 ? tuning().fwd_bands : 1; }
 
+// Chunked strip reads (tuning knob GT_TUNE_FWD_CHUNK = 4, bin groups only) read
+// 16 bytes per load and need both image copies with every row padded by
+// kRowPad zeros; otherwise an F-ordered input is read in place.
+constexpr int kRowPad = 4;
+static bool fwd_chunked() { return tuning().fwd_group > 1 && tuning().fwd_chunk > 1; }
+
 static size_t fwd_ws(int dtype, const Layout &L, int img_order) {
-  const size_t img = align_up(elem_size(dtype) * (size_t)L.Nx * L.Ny * L.Nz);
+  const size_t e = elem_size(dtype);
   size_t n = 0;
-  if (img_order != GT_ORDER_F) n += img;                  // x-fastest copy
-  if (!(img_order == GT_ORDER_C && L.Nz == 1)) n += img;  // y-fastest copy
+  if (fwd_chunked()) {
+    n += align_up(e * (size_t)(L.Nx + kRowPad) * L.Ny * L.Nz);  // x-fastest padded copy
+    n += align_up(e * (size_t)(L.Ny + kRowPad) * L.Nx * L.Nz);  // y-fastest padded copy
+  } else {
+    const size_t img = align_up(e * (size_t)L.Nx * L.Ny * L.Nz);
+    if (img_order != GT_ORDER_F) n += img;                  // x-fastest copy
+    if (!(img_order == GT_ORDER_C && L.Nz == 1)) n += img;  // y-fastest copy
+  }
   const int nb = fwd_bands();
-  if (nb > 1) n += align_up(elem_size(dtype) * (size_t)nb * L.Ns * L.Na * L.Nz);  // band partials
+  if (nb > 1) n += align_up(e * (size_t)nb * L.Ns * L.Na * L.Nz);  // band partials
   return n;
 }
 
@@ -390,6 +402,31 @@ static Job<real> make_job(bool fan, const Layout &L, const real *ofs, const real
   return J;
 }
 
+// Zeros columns [n0, n0 + kRowPad) of every row of a row-padded copy.
+template <typename real>
+__global__ __launch_bounds__(256) void pad_rows_kernel(real *__restrict__ dst, int64_t row,
+                                                       int64_t rows, int n0) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < rows * kRowPad) dst[(i / kRowPad) * row + n0 + i % kRowPad] = 0;
+}
+
+template <typename real, bool FAN, int K>
+static void launch_fwd_group(hipStream_t st, const Job<real> &J, dim3 grid, int nb, real *part) {
+  constexpr int CH = 16 / sizeof(real);
+  const dim3 block(64, 4);
+  const bool chunk = fwd_chunked();
+  if (nb > 1) {
+    if (chunk)
+      hipLaunchKernelGGL((fwd_band_kernel<real, FAN, K, CH>), grid, block, 0, st, J, part, nb);
+    else
+      hipLaunchKernelGGL((fwd_band_kernel<real, FAN, K, 1>), grid, block, 0, st, J, part, nb);
+  } else if (chunk) {
+    hipLaunchKernelGGL((fwd_kernel<real, FAN, K, CH>), grid, block, 0, st, J);
+  } else {
+    hipLaunchKernelGGL((fwd_kernel<real, FAN, K, 1>), grid, block, 0, st, J);
+  }
+}
+
 template <typename real, bool FAN>
 static int fwd_impl(hipStream_t st, real *sino, const real *img, const real *ofs,
                     const real *sdpd, const real *geo, const Layout &L, int sino_order,
@@ -398,55 +435,73 @@ static int fwd_impl(hipStream_t st, real *sino, const real *img, const real *ofs
   if (ws_bytes < need) return fail(GT_ENOSPC, "projection workspace too small");
   GT_REQUIRE(need == 0 || ws != nullptr, "workspace pointer is NULL");
   const Strides3 is = order_strides(img_order, L.Nx, L.Ny, L.Nz);
-  const Strides3 xs = {1, L.Nx, (int64_t)L.Nx * L.Ny};  // x fastest
-  const Strides3 ys = {L.Ny, 1, (int64_t)L.Nx * L.Ny};  // y fastest
   char *w = static_cast<char *>(ws);
-  const size_t img_bytes = align_up(sizeof(real) * (size_t)L.Nx * L.Ny * L.Nz);
   Job<real> J = make_job<real>(FAN, L, ofs, sdpd, geo);
   J.imgX = img;
   J.imgY = img;
-  if (img_order != GT_ORDER_F) {
-    real *cp = reinterpret_cast<real *>(w);
-    w += img_bytes;
-    launch_relayout<real>(st, cp, xs, img, is, L.Nx, L.Ny, L.Nz);
-    J.imgX = cp;
-  }
-  if (!(img_order == GT_ORDER_C && L.Nz == 1)) {
-    real *cp = reinterpret_cast<real *>(w);
-    launch_relayout<real>(st, cp, ys, img, is, L.Nx, L.Ny, L.Nz);
-    J.imgY = cp;
+  J.rowX = L.Nx;
+  J.rowY = L.Ny;
+  J.sliceX = J.sliceY = (int64_t)L.Nx * L.Ny;
+  if (fwd_chunked()) {
+    const int64_t rx = L.Nx + kRowPad, ry = L.Ny + kRowPad;
+    real *cx = reinterpret_cast<real *>(w);
+    w += align_up(sizeof(real) * (size_t)rx * L.Ny * L.Nz);
+    real *cy = reinterpret_cast<real *>(w);
+    launch_relayout<real>(st, cx, Strides3{1, rx, rx * L.Ny}, img, is, L.Nx, L.Ny, L.Nz);
+    launch_relayout<real>(st, cy, Strides3{ry, 1, ry * L.Nx}, img, is, L.Nx, L.Ny, L.Nz);
+    const int64_t nrx = (int64_t)L.Ny * L.Nz, nry = (int64_t)L.Nx * L.Nz;
+    hipLaunchKernelGGL(pad_rows_kernel<real>, dim3((unsigned)((nrx * kRowPad + 255) / 256)),
+                       dim3(256), 0, st, cx, rx, nrx, L.Nx);
+    hipLaunchKernelGGL(pad_rows_kernel<real>, dim3((unsigned)((nry * kRowPad + 255) / 256)),
+                       dim3(256), 0, st, cy, ry, nry, L.Ny);
+    J.imgX = cx;
+    J.imgY = cy;
+    J.rowX = rx;
+    J.rowY = ry;
+    J.sliceX = rx * L.Ny;
+    J.sliceY = ry * L.Nx;
+  } else {
+    const Strides3 xs = {1, L.Nx, (int64_t)L.Nx * L.Ny};  // x fastest
+    const Strides3 ys = {L.Ny, 1, (int64_t)L.Nx * L.Ny};  // y fastest
+    const size_t img_bytes = align_up(sizeof(real) * (size_t)L.Nx * L.Ny * L.Nz);
+    if (img_order != GT_ORDER_F) {
+      real *cp = reinterpret_cast<real *>(w);
+      w += img_bytes;
+      launch_relayout<real>(st, cp, xs, img, is, L.Nx, L.Ny, L.Nz);
+      J.imgX = cp;
+    }
+    if (!(img_order == GT_ORDER_C && L.Nz == 1)) {
+      real *cp = reinterpret_cast<real *>(w);
+      launch_relayout<real>(st, cp, ys, img, is, L.Nx, L.Ny, L.Nz);
+      J.imgY = cp;
+    }
   }
   J.sino = sino;
   J.so = order_strides(sino_order, L.Ns, L.Na, L.Nz);
   const int K = tuning().fwd_group;
-  const dim3 block(64, 4);
   const int groups = (L.Ns + 64 * K - 1) / (64 * K);
   const int nb = fwd_bands();
+  real *part = nullptr;
   if (nb > 1) {
     GT_REQUIRE((int64_t)groups * nb < ((int64_t)1 << 31), "too many detector groups");
     // the band partials are the last part of the workspace (after the image copies)
     const size_t part_bytes = align_up(sizeof(real) * (size_t)nb * L.Ns * L.Na * L.Nz);
-    real *part = reinterpret_cast<real *>(static_cast<char *>(ws) + (need - part_bytes));
-    const dim3 grid(groups * nb, (L.Na + 3) / 4, L.Nz);
-    if (K == 8)
-      hipLaunchKernelGGL((fwd_band_kernel<real, FAN, 8>), grid, block, 0, st, J, part, nb);
-    else if (K == 4)
-      hipLaunchKernelGGL((fwd_band_kernel<real, FAN, 4>), grid, block, 0, st, J, part, nb);
-    else
-      hipLaunchKernelGGL((fwd_band_kernel<real, FAN, 2>), grid, block, 0, st, J, part, nb);
+    part = reinterpret_cast<real *>(static_cast<char *>(ws) + (need - part_bytes));
+  }
+  const dim3 grid(groups * nb, (L.Na + 3) / 4, L.Nz);
+  if (K == 16)
+    launch_fwd_group<real, FAN, 16>(st, J, grid, nb, part);
+  else if (K == 8)
+    launch_fwd_group<real, FAN, 8>(st, J, grid, nb, part);
+  else if (K == 4)
+    launch_fwd_group<real, FAN, 4>(st, J, grid, nb, part);
+  else if (K == 2)
+    launch_fwd_group<real, FAN, 2>(st, J, grid, nb, part);
+  else
+    hipLaunchKernelGGL((fwd_kernel<real, FAN, 1>), grid, dim3(64, 4), 0, st, J);
+  if (nb > 1)
     hipLaunchKernelGGL(band_reduce_kernel<real>, dim3((L.Ns + 255) / 256, L.Na, L.Nz), dim3(256),
                        0, st, J, part, nb);
-    return check_launch(FAN ? "fan forward projection" : "radon forward projection");
-  }
-  const dim3 grid(groups, (L.Na + 3) / 4, L.Nz);
-  if (K == 8)
-    hipLaunchKernelGGL((fwd_kernel<real, FAN, 8>), grid, block, 0, st, J);
-  else if (K == 4)
-    hipLaunchKernelGGL((fwd_kernel<real, FAN, 4>), grid, block, 0, st, J);
-  else if (K == 2)
-    hipLaunchKernelGGL((fwd_kernel<real, FAN, 2>), grid, block, 0, st, J);
-  else
-    hipLaunchKernelGGL((fwd_kernel<real, FAN, 1>), grid, block, 0, st, J);
   return check_launch(FAN ? "fan forward projection" : "radon forward projection");
 }
 

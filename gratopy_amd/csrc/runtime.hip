@@ -35,8 +35,8 @@ extern "C" int gt_tune_set(int key, int value) {
   gt::Tuning &t = gt::tuning();
   switch (key) {
     case GT_TUNE_FWD_GROUP:
-      if (value != 1 && value != 2 && value != 4 && value != 8)
-        return gt::fail(GT_EINVAL, "forward group size must be 1, 2, 4 or 8");
+      if (value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
+        return gt::fail(GT_EINVAL, "forward group size must be 1, 2, 4, 8 or 16");
       t.fwd_group = value;
       return GT_OK;
     case GT_TUNE_BACK_ROWS:
@@ -58,6 +58,11 @@ extern "C" int gt_tune_set(int key, int value) {
       if (value != 0 && value != 8 && value != 16)
         return gt::fail(GT_EINVAL, "backprojection staging chunk must be 0, 8 or 16 angles");
       t.back_stage = value;
+      return GT_OK;
+    case GT_TUNE_FWD_CHUNK:
+      if (value != 1 && value != 4)
+        return gt::fail(GT_EINVAL, "forward chunk must be 1 or 4");
+      t.fwd_chunk = value;
       return GT_OK;
     case GT_TUNE_BACK_TILE:
       if (value != 16 && value != 32)
@@ -84,6 +89,8 @@ extern "C" int gt_tune_get(int key) {
       return t.back_stage;
     case GT_TUNE_BACK_TILE:
       return t.back_tile;
+    case GT_TUNE_FWD_CHUNK:
+      return t.fwd_chunk;
     default:
       return -1;
   }

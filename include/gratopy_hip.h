@@ -44,7 +44,7 @@ int gt_version(void);
  * No reference counterpart: the reference leaves local sizes to the OpenCL
  * driver (gratopy/gratopy.py:304-313).  Results do not depend on them beyond
  * rounding.
- *   GT_TUNE_FWD_GROUP: detector bins per lane in the forward projection (1, 2, 4, 8)
+ *   GT_TUNE_FWD_GROUP: detector bins per lane in the forward projection (1, 2, 4, 8, 16)
  *   GT_TUNE_BACK_ROWS: pixels per lane in the backprojection (1, 2, 4, 8)
  *   GT_TUNE_FWD_BANDS: row bands of the forward strip walk (1, 2, 4, 8; applies when
  *                      GT_TUNE_FWD_GROUP > 1).  With B > 1 bands, consecutive thread
@@ -60,14 +60,19 @@ int gt_version(void);
  *                      interpolation taps from there.
  *   GT_TUNE_BACK_TILE: lanes along x of a staged backprojection tile (16 or 32; the
  *                      tile is that many pixels wide and 256 / width * rows-per-lane
- *                      pixels tall). */
+ *                      pixels tall).
+ *   GT_TUNE_FWD_CHUNK: 4 = the forward bin groups read their strip rows 16 bytes per
+ *                      load (4 float32 or 2 float64 pixels) from row-padded copies of
+ *                      the image (the workspace then holds two padded copies); 1 = one
+ *                      pixel per load. */
 enum {
   GT_TUNE_FWD_GROUP = 0,
   GT_TUNE_BACK_ROWS = 1,
   GT_TUNE_FWD_BANDS = 2,
   GT_TUNE_BACK_BANDS = 3,
   GT_TUNE_BACK_STAGE = 4,
-  GT_TUNE_BACK_TILE = 5
+  GT_TUNE_BACK_TILE = 5,
+  GT_TUNE_FWD_CHUNK = 6
 };
 int gt_tune_set(int key, int value);
 int gt_tune_get(int key);

@@ -1,7 +1,8 @@
 """Every launch variant the autotuner can pick (forward bin groups K = 1, 2,
-4, 8, with or without 8 row bands; backprojection rows per lane RY = 1, 2, 4, 8,
-with or without 8 angle bands, or with sinogram windows staged in LDS for
-chunks of 8 or 16 angles in tiles 16 or 32 pixels wide)
+4, 8, with or without 8 row bands, reading strip rows one pixel or 16 bytes
+per load; backprojection rows per lane RY = 1, 2, 4, 8, with or without 8
+angle bands, or with sinogram windows staged in LDS for chunks of 8 or 16
+angles in tiles 16 or 32 pixels wide)
 meets the parity bar against
 the oracle: normwise 1e-5 in float32, 1e-12 in float64, on regular, shifted,
 ragged and degenerate geometries (including fan bins so wide that a bin
@@ -35,14 +36,17 @@ CASES = [
 def restore_variant():
     before = autotune.current_variant()
     yield
-    autotune.set_variant(before["fwd_group"], before["back_rows"], before["fwd_bands"],
-                         before["back_bands"], before["back_stage"], before["back_tile"])
+    autotune.set_variant(**before)
 
 
-@pytest.mark.parametrize("K,bands", autotune.FWD_VARIANTS)
-def test_forward_variants(K, bands, restore_variant):
-    print(f"variant K={K} bands={bands}", flush=True)
-    autotune.set_variant(fwd_group=K, fwd_bands=bands)
+FWD_IDS = [autotune.variant_key(v) for v in autotune.FWD_VARIANTS]
+BACK_IDS = [autotune.variant_key(v) for v in autotune.BACK_VARIANTS]
+
+
+@pytest.mark.parametrize("variant", autotune.FWD_VARIANTS, ids=FWD_IDS)
+def test_forward_variants(variant, restore_variant):
+    print(f"variant {autotune.variant_key(variant)}", flush=True)
+    autotune.set_variant(**variant)
     for geometry, shape, na, nd, kw in CASES:
         g = gratopy.RADON if geometry == "radon" else gratopy.FANBEAM
         ps = gratopy.ProjectionSettings("cuda:0", g, shape, na, nd, **kw)
@@ -51,13 +55,13 @@ def test_forward_variants(K, bands, restore_variant):
             ref = _oracle(ps, img, dtype, False)
             got = _hip(ps, img, dtype, False)
             scale = max(np.abs(ref).max(), 1e-3)
-            assert np.abs(got - ref).max() <= TOL[dtype] * scale, (geometry, shape, K, bands, dtype)
+            assert np.abs(got - ref).max() <= TOL[dtype] * scale, (geometry, shape, variant, dtype)
 
 
-@pytest.mark.parametrize("RY,bands,stage,tile", autotune.BACK_VARIANTS)
-def test_backward_variants(RY, bands, stage, tile, restore_variant):
-    print(f"variant RY={RY} bands={bands} stage={stage} tile={tile}", flush=True)
-    autotune.set_variant(back_rows=RY, back_bands=bands, back_stage=stage, back_tile=tile)
+@pytest.mark.parametrize("variant", autotune.BACK_VARIANTS, ids=BACK_IDS)
+def test_backward_variants(variant, restore_variant):
+    print(f"variant {autotune.variant_key(variant)}", flush=True)
+    autotune.set_variant(**variant)
     for geometry, shape, na, nd, kw in CASES:
         g = gratopy.RADON if geometry == "radon" else gratopy.FANBEAM
         ps = gratopy.ProjectionSettings("cuda:0", g, shape, na, nd, **kw)
@@ -66,8 +70,7 @@ def test_backward_variants(RY, bands, stage, tile, restore_variant):
             ref = _oracle(ps, sino, dtype, True)
             got = _hip(ps, sino, dtype, True, in_order="C", out_order="C")
             scale = max(np.abs(ref).max(), 1e-3)
-            assert np.abs(got - ref).max() <= TOL[dtype] * scale, (geometry, shape, RY, bands, stage,
-                                                                   tile, dtype)
+            assert np.abs(got - ref).max() <= TOL[dtype] * scale, (geometry, shape, variant, dtype)
 
 
 _LARGE = {}
@@ -90,20 +93,21 @@ def _large_case(geometry):
     return _LARGE[geometry]
 
 
-@pytest.mark.parametrize("K,bands", autotune.FWD_VARIANTS)
+@pytest.mark.parametrize("variant", autotune.FWD_VARIANTS, ids=FWD_IDS)
 @pytest.mark.parametrize("geometry", ["radon", "fan"])
-def test_forward_variants_large_float32(geometry, K, bands, restore_variant):
+def test_forward_variants_large_float32(geometry, variant, restore_variant):
     """At 768 detectors the float32 result of every variant is within 1e-5 of
     the float64 oracle, or within 1.2x the distance of the reference
     algorithm in float32 (the oracle in float32).  At this size both sit at
     the float32 floor (~1.4e-5 for Radon): the float32 geometry tables alone
     put ~3e-5 detector bins of error into x*X + y*Y + off."""
-    print(f"variant K={K} bands={bands} large {geometry}", flush=True)
+    name = autotune.variant_key(variant)
+    print(f"variant {name} large {geometry}", flush=True)
     ps, img, truth, ref32 = _large_case(geometry)
-    autotune.set_variant(fwd_group=K, fwd_bands=bands)
+    autotune.set_variant(**variant)
     got = _hip(ps, img, F32, False)
     e_hip, e_ref = rel_err(got, truth), rel_err(ref32, truth)
-    print(f"K={K} bands={bands} {geometry}: float32 error vs float64 oracle: hip {e_hip:.3e}, "
+    print(f"{name} {geometry}: float32 error vs float64 oracle: hip {e_hip:.3e}, "
           f"reference algorithm {e_ref:.3e}", flush=True)
     assert e_hip <= max(TOL[F32], 1.2 * e_ref)
 
@@ -118,14 +122,13 @@ def test_autotuner_picks_a_variant_and_caches_it(tmp_path, monkeypatch, restore_
     sino = gratopy.forwardprojection(img, ps)
     best = autotune.tune(ps, img.clone(), sino.clone(), repeats=2)
     assert best["fwd_group"] in autotune.FWD_GROUPS and best["back_rows"] in autotune.BACK_ROWS
-    assert autotune.current_variant() == {"fwd_group": best["fwd_group"], "back_rows": best["back_rows"],
-                                          "fwd_bands": best["fwd_bands"], "back_bands": best["back_bands"],
-                                          "back_stage": best["back_stage"], "back_tile": best["back_tile"]}
-    autotune.set_variant(1, 1, 1, 1)
+    assert autotune.current_variant() == {k: best[k] for k in autotune.KNOBS}
+    assert set(best["fwd_ms"]) == set(FWD_IDS) and set(best["back_ms"]) == set(BACK_IDS)
+    autotune.set_variant(**autotune.DEFAULTS)
     assert autotune.apply(ps, F32, 1)
-    assert autotune.current_variant()["fwd_group"] == best["fwd_group"]
+    assert autotune.current_variant() == {k: best[k] for k in autotune.KNOBS}
     # all variants agree to rounding on the tuned geometry
     ref = dev.to_host(gratopy.forwardprojection(img, ps))
-    for K, bands in autotune.FWD_VARIANTS:
-        autotune.set_variant(fwd_group=K, fwd_bands=bands)
-        assert rel_err(dev.to_host(gratopy.forwardprojection(img, ps)), ref) < 1e-5
+    for variant in autotune.FWD_VARIANTS:
+        autotune.set_variant(**variant)
+        assert rel_err(dev.to_host(gratopy.forwardprojection(img, ps)), ref) < 1e-5, variant
