@@ -151,14 +151,13 @@ def uneven_mix(n=64, seed=3):
     return out
 
 
-def bench_uneven_landweber(n_iter=100):
-    """Landweber(100) on the 64-slice uneven mix: one batched plan, graph
-    captured, vs the plain per-slice loop of landweber calls."""
-    import torch
-
+def uneven_problem():
+    """The 64-slice uneven mix with noisy sinograms, norm estimates, the
+    pixel-angle work of one fwd+back over all slices, and the batch
+    autotuner's choice (installed)."""
     import gratopy_amd as gratopy
     from gratopy_amd import _device as dev
-    from gratopy_amd.batch import batch_landweber
+    from gratopy_amd import autotune
 
     mix = uneven_mix()
     settings = [ps for ps, _ in mix]
@@ -169,12 +168,21 @@ def bench_uneven_landweber(n_iter=100):
         sinos.append(dev.to_device(np.asfortranarray(add_noise(s, 0.5, rng)), "cuda:0"))
     norms = [gratopy.normest(ps) for ps in settings]
     work = sum(ps.img_shape[0] * ps.img_shape[1] * ps.sinogram_shape[1] for ps in settings)
-    from gratopy_amd import autotune
-
     scratch_i = [dev.empty(ps.img_shape, np.float32, order="F", device="cuda:0") for ps in settings]
     scratch_s = [dev.empty(ps.sinogram_shape, np.float32, order="F", device="cuda:0") for ps in settings]
     tuned = autotune.tune_batch(settings, scratch_i, scratch_s)
-    del scratch_i, scratch_s
+    return settings, sinos, norms, work, tuned
+
+
+def bench_uneven_landweber(problem, n_iter=100):
+    """Landweber(100) on the 64-slice uneven mix: one batched plan, graph
+    captured, vs the plain per-slice loop of landweber calls."""
+    import torch
+
+    import gratopy_amd as gratopy
+    from gratopy_amd.batch import batch_landweber
+
+    settings, sinos, norms, work, tuned = problem
     run_b = lambda: batch_landweber(sinos, settings, n_iter, norm_estimates=norms, graph=True)
     run_p = lambda: [gratopy.landweber(f, ps, n_iter, norm_estimate=n)
                      for f, ps, n in zip(sinos, settings, norms)]
@@ -188,7 +196,33 @@ def bench_uneven_landweber(n_iter=100):
             "batched_graph": round(n / tb, 3), "plain_per_slice": round(n / tp, 3),
             "speedup": round(tp / tb, 3), "fwd+back GUPS batched": round(work * n_iter / tb / 1e9, 2),
             "bitwise_equal": same,
-            "tuning": {"fwd_group": tuned["fwd_group"], "back_rows": tuned["back_rows"], "source": "autotuner"}}
+            "tuning": {**{k: tuned[k] for k in ("fwd_group", "fwd_bands", "fwd_chunk", "back_rows",
+                                                "back_stage", "back_tile")}, "source": "autotuner"}}
+
+
+def bench_uneven_tv(problem, n_iter=100, mu=1e-2):
+    """TV(100) (mu = 1e-2) on the 64-slice uneven mix: one batched plan, graph
+    captured, vs the plain per-slice loop of total_variation calls (the
+    reference's structure: seven launches per iteration with host scalars)."""
+    import torch
+
+    import gratopy_amd as gratopy
+    from gratopy_amd.batch import batch_total_variation
+
+    settings, sinos, norms, work, _ = problem
+    run_b = lambda: batch_total_variation(sinos, settings, mu, n_iter, norm_estimates=norms, graph=True)
+    run_p = lambda: [gratopy.total_variation(f, ps, mu, n_iter, norm_estimate=n)
+                     for f, ps, n in zip(sinos, settings, norms)]
+    run_b()
+    tb, ub = _timed(run_b)
+    run_p()
+    tp, up = _timed(run_p)
+    same = all(torch.equal(a, b) for a, b in zip(ub, up))
+    n = len(settings)
+    return {"metric": "TV(100) slices/s, 64 uneven slices (config of 128..1024 px, 60..720 angles)",
+            "batched_graph": round(n / tb, 3), "plain_per_slice": round(n / tp, 3),
+            "speedup": round(tp / tb, 3), "fwd+back GUPS batched": round(work * n_iter / tb / 1e9, 2),
+            "bitwise_equal": same}
 
 
 def measured_traffic(workload, kernel):
@@ -371,8 +405,11 @@ def main():
         print("tv stack measured", file=sys.stderr, flush=True)
         del img
         torch.cuda.empty_cache()
-        out["batched"]["landweber_uneven"] = bench_uneven_landweber()
+        problem = uneven_problem()
+        out["batched"]["landweber_uneven"] = bench_uneven_landweber(problem)
         print("uneven landweber measured", file=sys.stderr, flush=True)
+        out["batched"]["tv_uneven"] = bench_uneven_tv(problem)
+        print("uneven tv measured", file=sys.stderr, flush=True)
     print(json.dumps(out), file=json_out, flush=True)
 
 

@@ -9,7 +9,7 @@ RY pixels along y to a lane (``back_rows`` in 1, 2, 4, 8) and either sums all
 angles per tile, splits them into 8 angle bands (``back_bands``), or stages
 each tile's sinogram windows in LDS for chunks of 8 or 16 angles
 (``back_stage``) in tiles 16 or 32 pixels wide (``back_tile``).
-Batched launches use K and RY only.  :func:`tune` times every variant on the
+Batched launches take all but the angle bands.  :func:`tune` times every variant on the
 caller's own geometry and array shapes with HIP events, keeps the fastest,
 and remembers it per shape key in a JSON cache
 (``$GRATOPY_AMD_TUNE_CACHE``, default ``~/.cache/gratopy_amd/tune.json``).
@@ -169,14 +169,15 @@ def tune(projectionsetting, img, sino, repeats: int = 3, store: bool = True) -> 
 
 
 def tune_batch(settings, imgs, sinos, repeats: int = 2) -> dict:
-    """Time all variants on a batch (the tile lists depend on the variant, so
-    one plan is built per variant) and install the fastest.  ``imgs`` and
-    ``sinos`` are overwritten.  Batched launches use neither bands, chunked
-    reads nor LDS staging, so those are switched off: single-slice calls then
-    run the same kernels as the batch."""
+    """Time the batch variants (forward K, row bands and chunked reads;
+    backprojection RY, unstaged or LDS-staged) on a batch and install the
+    fastest.  The tile lists depend on the variant, so one plan is built per
+    variant.  ``imgs`` and ``sinos`` are overwritten.  Batched backprojections
+    use no angle bands, so those are switched off: single-slice calls then run
+    the same kernels as the batch."""
     from .batch import BatchPlan
 
-    set_variant(fwd_bands=1, back_bands=1, back_stage=0, fwd_chunk=1)
+    set_variant(back_bands=1)
 
     def timed(op):
         plan = BatchPlan(settings, imgs, sinos)
@@ -186,16 +187,18 @@ def tune_batch(settings, imgs, sinos, repeats: int = 2) -> dict:
             plan.destroy()
 
     fwd_times, back_times = {}, {}
-    for k in FWD_GROUPS:
-        set_variant(fwd_group=k)
-        fwd_times[k] = timed(_hip.BATCH_FORWARD)
-    for r in BACK_ROWS:
-        set_variant(back_rows=r)
-        back_times[r] = timed(_hip.BATCH_BACKWARD)
-    best = {"fwd_group": min(fwd_times, key=fwd_times.get), "back_rows": min(back_times, key=back_times.get),
-            "fwd_ms": fwd_times, "back_ms": back_times}
-    set_variant(fwd_group=best["fwd_group"], back_rows=best["back_rows"])
-    return best
+    for v in FWD_VARIANTS:
+        set_variant(**v)
+        fwd_times[variant_key(v)] = (timed(_hip.BATCH_FORWARD), v)
+    for v in BACK_VARIANTS:
+        if v["back_bands"] == 1:
+            set_variant(**v)
+            back_times[variant_key(v)] = (timed(_hip.BATCH_BACKWARD), v)
+    knobs = {**min(fwd_times.values(), key=lambda tv: tv[0])[1],
+             **min(back_times.values(), key=lambda tv: tv[0])[1]}
+    set_variant(**knobs)
+    return {**knobs, "fwd_ms": {k: t for k, (t, _) in fwd_times.items()},
+            "back_ms": {k: t for k, (t, _) in back_times.items()}}
 
 
 def apply(projectionsetting, dtype, nz: int = 1) -> bool:

@@ -7,8 +7,9 @@
 //   * job records (struct Job of kernels.h), one per slice and table kind
 //     (plain projections, Landweber, TV even / odd iteration);
 //   * the per-angle backprojection tables of every slice (built once);
-//   * per-slice scratch: a y-fastest copy of the image and the padded,
-//     pre-weighted sinogram of the backprojection;
+//   * per-slice scratch: a y-fastest copy of the image (and, for chunked
+//     strip reads, a row-padded x-fastest copy; rows of both then end in
+//     zeros) and the padded, pre-weighted sinogram of the backprojection;
 //   * tile lists: one entry (slice, tile coordinates) per thread block of
 //     each launch kind, sorted by estimated cost (largest first) so the
 //     hardware dispatcher, which hands out blocks in order as slots free,
@@ -27,6 +28,9 @@ struct gt_batch_plan {
   int dtype;
   int n_items;
   int fwd_group, back_rows;
+  int fwd_bands;              // > 1: row-banded forward (per-slice partials, ordered reduction)
+  int fwd_chunk;              // 4: forward strip rows read from row-padded copies, 16 bytes per load
+  int back_stage, back_tile;  // > 0: LDS-staged backprojection (angles per chunk, tile width)
   bool has_data;  // every item has data (Landweber, TV)
   bool has_tv;    // every item has data, img2 and field (TV)
   void *mem;  // one hipMalloc: jobs, tiles, tables, scratch
@@ -41,18 +45,24 @@ namespace gt {
 enum TileKind { kRelayout = 0, kForward = 1, kStage = 2, kBackward = 3, kDual = 4 };
 enum JobKind { kPlain = 0, kLandweber = 1, kTvEven = 2, kTvOdd = 3 };
 
+// tile (slice, 32x32 block): the forward's image copies of src (the y-fastest
+// one, and the row-padded x-fastest one when imgX is not src itself)
 template <typename real>
 __global__ __launch_bounds__(256) void batch_relayout_kernel(const Job<real> *__restrict__ jobs,
                                                              const int4 *__restrict__ tiles) {
   __shared__ real tile[32][33];
   const int4 t = tiles[blockIdx.x];
   const Job<real> &J = jobs[t.x];
-  const Strides3 xs = {1, J.Nx, (int64_t)J.Nx * J.Ny};
-  const Strides3 ys = {J.Ny, 1, (int64_t)J.Nx * J.Ny};
-  relayout_tile(tile, const_cast<real *>(J.imgY), ys, J.imgX, xs, J.Nx, J.Ny, 0, t.y, t.z);
+  const Strides3 ss = {1, J.Nx, (int64_t)J.Nx * J.Ny};
+  const Strides3 ys = {J.rowY, 1, J.sliceY};
+  relayout_tile(tile, const_cast<real *>(J.imgY), ys, J.src, ss, J.Nx, J.Ny, 0, t.y, t.z);
+  if (J.imgX != J.src) {
+    const Strides3 xs = {1, J.rowX, J.sliceX};
+    relayout_tile(tile, const_cast<real *>(J.imgX), xs, J.src, ss, J.Nx, J.Ny, 0, t.y, t.z);
+  }
 }
 
-template <typename real, int K>
+template <typename real, int K, int CH = 1>
 __global__ __launch_bounds__(256) void batch_fwd_kernel(const Job<real> *__restrict__ jobs,
                                                         const int4 *__restrict__ tiles) {
   const int4 t = tiles[blockIdx.x];
@@ -71,10 +81,39 @@ __global__ __launch_bounds__(256) void batch_fwd_kernel(const Job<real> *__restr
     if (s0 >= J.Ns || a >= J.Na) return;
     real *acc = slots + threadIdx.y * 64 + threadIdx.x;
     if (J.fan)
-      fan_fwd_group<real, K>(J, s0, a, 0, acc);
+      fan_fwd_group<real, K, false, CH>(J, s0, a, 0, acc);
     else
-      radon_fwd_group<real, K>(J, s0, a, 0, acc);
+      radon_fwd_group<real, K, false, CH>(J, s0, a, 0, acc);
   }
+}
+
+// tile (slice, group block, angle block, band): one row band of the strip walks
+// of 64 x 4 bin groups; raw sums go to the slice's band partials
+template <typename real, int K, int CH, int NB>
+__global__ __launch_bounds__(256) void batch_fwd_band_kernel(const Job<real> *__restrict__ jobs,
+                                                             const int4 *__restrict__ tiles) {
+  __shared__ real slots[Slots<K>::kCount * 256];
+  const int4 t = tiles[blockIdx.x];
+  const Job<real> &J = jobs[t.x];
+  const int a = t.z * 4 + threadIdx.y;
+  const int s0 = (t.y * 64 + threadIdx.x) * K;
+  if (s0 >= J.Ns || a >= J.Na) return;
+  real *acc = slots + threadIdx.y * 64 + threadIdx.x;
+  const Band<real> B{t.w, NB, J.part};
+  if (J.fan)
+    fan_fwd_group<real, K, true, CH>(J, s0, a, 0, acc, B);
+  else
+    radon_fwd_group<real, K, true, CH>(J, s0, a, 0, acc, B);
+}
+
+// tile (slice, angle): the bands of one sinogram row added in band order, then
+// the forward's scale and epilogue
+template <typename real, int NB>
+__global__ __launch_bounds__(256) void batch_band_reduce_kernel(const Job<real> *__restrict__ jobs,
+                                                                const int4 *__restrict__ tiles) {
+  const int4 t = tiles[blockIdx.x];
+  const Job<real> &J = jobs[t.x];
+  for (int s = threadIdx.x; s < J.Ns; s += 256) band_reduce_entry(J, J.part, NB, s, t.y, 0);
 }
 
 // tile (slice, angle): one padded sinogram row
@@ -100,6 +139,20 @@ __global__ __launch_bounds__(256) void batch_back_kernel(const Job<real> *__rest
     back_pixels<real, true, RY>(J, x, y0, 0);
   else
     back_pixels<real, false, RY>(J, x, y0, 0);
+}
+
+// tile (slice, tile x, tile y): LDS-staged backprojection of TX x (256/TX)*RY pixels
+template <typename real, int RY, int CA, int TX>
+__global__ __launch_bounds__(256) void batch_back_staged_kernel(const Job<real> *__restrict__ jobs,
+                                                                const int4 *__restrict__ tiles) {
+  __shared__ real rows[CA * kStageWidth];
+  __shared__ StageWindow<real> win[CA];
+  const int4 t = tiles[blockIdx.x];
+  const Job<real> &J = jobs[t.x];
+  if (J.fan)
+    back_staged_tile<real, true, RY, CA, TX>(J, t.y, t.z, 0, rows, win);
+  else
+    back_staged_tile<real, false, RY, CA, TX>(J, t.y, t.z, 0, rows, win);
 }
 
 // tile (slice, first pixel): 256 pixels of the dual update
@@ -141,8 +194,8 @@ static int check_item(const gt_batch_item &it) {
 struct TileSet {
   std::vector<int4> t;
   std::vector<double> cost;
-  void add(int item, int a, int b, double c) {
-    t.push_back(make_int4(item, a, b, 0));
+  void add(int item, int a, int b, double c, int w = 0) {
+    t.push_back(make_int4(item, a, b, w));
     cost.push_back(c);
   }
   // largest first; ties keep slice order
@@ -156,9 +209,16 @@ struct TileSet {
   }
 };
 
+// Rows of the forward's padded image copies end in this many zeros (chunked reads).
+constexpr int kBatchRowPad = 4;
+
 template <typename real>
 static int plan_create(gt_batch_plan **out, int dtype, int n, const gt_batch_item *items, int K,
-                       int RY) {
+                       int RY, int nb, int chunk, int stage, int tx) {
+  const bool chunked = K > 1 && chunk > 1;
+  nb = K > 1 && nb > 1 ? nb : 1;
+  const int pad = chunked ? kBatchRowPad : 0;
+  const int ty = 256 / tx;  // lane rows of a staged tile
   // ---- layout of the single allocation ----
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -167,13 +227,18 @@ static int plan_create(gt_batch_plan **out, int dtype, int n, const gt_batch_ite
     return at;
   };
   const size_t jobs_at = take(4 * sizeof(Job<real>) * (size_t)n);
-  std::vector<size_t> tab_at(n), S_at(n), Y_at(n);
+  std::vector<size_t> tab_at(n), S_at(n), Y_at(n), X_at(n), part_at(n);
+  const size_t copies_from = off;
   for (int i = 0; i < n; ++i) {
     const gt_batch_item &it = items[i];
     tab_at[i] = take(sizeof(AngleRec<real>) * (size_t)it.Na);
     S_at[i] = take(sizeof(real) * (size_t)(it.Ns + 4) * it.Na);
-    Y_at[i] = take(sizeof(real) * (size_t)it.Nx * it.Ny);
+    Y_at[i] = take(sizeof(real) * (size_t)(it.Ny + pad) * it.Nx);
+    if (chunked) X_at[i] = take(sizeof(real) * (size_t)(it.Nx + pad) * it.Ny);
   }
+  const size_t copies_to = off;
+  if (nb > 1)
+    for (int i = 0; i < n; ++i) part_at[i] = take(sizeof(real) * (size_t)nb * items[i].Ns * items[i].Na);
   TileSet sets[5];
   for (int i = 0; i < n; ++i) {
     const gt_batch_item &it = items[i];
@@ -181,14 +246,17 @@ static int plan_create(gt_batch_plan **out, int dtype, int n, const gt_batch_ite
     // forward tile: ~ rows crossed x strip width per ray group
     const double cf =
         (double)std::max(it.Nx, it.Ny) * (K + 1) * (it.geometry == GT_GEOMETRY_FAN ? 1.5 : 1.0);
+    // bands innermost: the 8 bands of a tile are dispatched to the 8 XCDs in turn
     for (int b = 0; b < nsb; ++b)
-      for (int a = 0; a < nab; ++a) sets[kForward].add(i, b, a, cf);
+      for (int a = 0; a < nab; ++a)
+        for (int w = 0; w < nb; ++w) sets[kForward].add(i, b, a, cf / nb, w);
     for (int bx = 0; bx < (it.Nx + 31) / 32; ++bx)
       for (int by = 0; by < (it.Ny + 31) / 32; ++by) sets[kRelayout].add(i, bx * 32, by * 32, 1.0);
     for (int a = 0; a < it.Na; ++a) sets[kStage].add(i, a, 0, (double)it.Ns);
     const double cb = (double)it.Na * RY * (it.geometry == GT_GEOMETRY_FAN ? 1.5 : 1.0);
-    for (int bx = 0; bx < (it.Nx + 15) / 16; ++bx)
-      for (int by = 0; by < (it.Ny + 16 * RY - 1) / (16 * RY); ++by) sets[kBackward].add(i, bx, by, cb);
+    const int tw = stage ? tx : 16, th = (stage ? ty : 16) * RY;  // back tile in pixels
+    for (int bx = 0; bx < (it.Nx + tw - 1) / tw; ++bx)
+      for (int by = 0; by < (it.Ny + th - 1) / th; ++by) sets[kBackward].add(i, bx, by, cb);
     const int64_t npix = (int64_t)it.Nx * it.Ny;
     for (int64_t p = 0; p < (npix + 255) / 256; ++p) sets[kDual].add(i, (int)p, 0, 1.0);
   }
@@ -204,6 +272,12 @@ static int plan_create(gt_batch_plan **out, int dtype, int n, const gt_batch_ite
     return fail(GT_EHIP, "batch plan: device allocation failed");
   }
   char *base = static_cast<char *>(mem);
+  // the image copies start as zeros: their row padding is never written again
+  if (hipMemset(base + copies_from, 0, copies_to - copies_from) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(mem);
+    return fail(GT_EHIP, "batch plan: clearing the image copies failed");
+  }
 
   // ---- job records ----
   std::vector<Job<real>> jobs(4 * (size_t)n);
@@ -222,9 +296,12 @@ static int plan_create(gt_batch_plan **out, int dtype, int n, const gt_batch_ite
     J.sino = static_cast<real *>(it.sino);
     J.so = order_strides(GT_ORDER_F, it.Ns, it.Na, 1);
     J.imgY = reinterpret_cast<const real *>(base + Y_at[i]);
-    J.rowX = it.Nx;
-    J.rowY = it.Ny;
-    J.sliceX = J.sliceY = (int64_t)it.Nx * it.Ny;
+    const real *xcopy = chunked ? reinterpret_cast<const real *>(base + X_at[i]) : nullptr;
+    J.rowX = it.Nx + pad;
+    J.rowY = it.Ny + pad;
+    J.sliceX = J.rowX * it.Ny;
+    J.sliceY = J.rowY * it.Nx;
+    J.part = nb > 1 ? reinterpret_cast<real *>(base + part_at[i]) : nullptr;
     J.io = order_strides(GT_ORDER_F, it.Nx, it.Ny, 1);
     J.S = reinterpret_cast<const real *>(base + S_at[i]);
     J.tab = reinterpret_cast<const AngleRec<real> *>(base + tab_at[i]);
@@ -239,18 +316,21 @@ static int plan_create(gt_batch_plan **out, int dtype, int n, const gt_batch_ite
 
     Job<real> P = J;  // plain: sino = R img, img = B sino
     P.mode = 0;
-    P.imgX = img;
+    P.src = img;
+    P.imgX = chunked ? xcopy : img;
     P.img = img;
     Job<real> L = P;  // Landweber: sino = R img - data; img += p0 * B sino
     L.mode = 1;
     L.scale = (real)it.p[0];
     Job<real> T0 = J;  // TV, even step: u_bar in img2, u in img; u' -> img2, u_bar' -> img
     T0.mode = 2;
-    T0.imgX = img2;
+    T0.src = img2;
+    T0.imgX = chunked ? xcopy : img2;
     T0.img = img2;
     T0.u = img;
     Job<real> T1 = T0;  // TV, odd step: roles swapped
-    T1.imgX = img;
+    T1.src = img;
+    T1.imgX = chunked ? xcopy : img;
     T1.img = img;
     T1.u = img2;
     jobs[kPlain * (size_t)n + i] = P;
@@ -282,6 +362,10 @@ static int plan_create(gt_batch_plan **out, int dtype, int n, const gt_batch_ite
   p->n_items = n;
   p->fwd_group = K;
   p->back_rows = RY;
+  p->fwd_bands = nb;
+  p->fwd_chunk = chunked ? chunk : 1;
+  p->back_stage = stage;
+  p->back_tile = tx;
   p->has_data = true;
   p->has_tv = true;
   for (int i = 0; i < n; ++i) {
@@ -299,6 +383,69 @@ static int plan_create(gt_batch_plan **out, int dtype, int n, const gt_batch_ite
   return GT_OK;
 }
 
+template <typename real, int K, int CH>
+static void launch_fwd_banded(const gt_batch_plan *p, hipStream_t st, const Job<real> *jobs,
+                              dim3 grid, dim3 block) {
+  switch (p->fwd_bands) {
+    case 8:
+      hipLaunchKernelGGL((batch_fwd_band_kernel<real, K, CH, 8>), grid, block, 0, st, jobs,
+                         p->tiles[kForward]);
+      hipLaunchKernelGGL((batch_band_reduce_kernel<real, 8>), dim3(p->n_tiles[kStage]), dim3(256), 0,
+                         st, jobs, p->tiles[kStage]);
+      break;
+    case 4:
+      hipLaunchKernelGGL((batch_fwd_band_kernel<real, K, CH, 4>), grid, block, 0, st, jobs,
+                         p->tiles[kForward]);
+      hipLaunchKernelGGL((batch_band_reduce_kernel<real, 4>), dim3(p->n_tiles[kStage]), dim3(256), 0,
+                         st, jobs, p->tiles[kStage]);
+      break;
+    default:
+      hipLaunchKernelGGL((batch_fwd_band_kernel<real, K, CH, 2>), grid, block, 0, st, jobs,
+                         p->tiles[kForward]);
+      hipLaunchKernelGGL((batch_band_reduce_kernel<real, 2>), dim3(p->n_tiles[kStage]), dim3(256), 0,
+                         st, jobs, p->tiles[kStage]);
+  }
+}
+
+template <typename real, int K>
+static void launch_fwd_k(const gt_batch_plan *p, hipStream_t st, const Job<real> *jobs, dim3 grid,
+                         dim3 block) {
+  constexpr int CH = 16 / sizeof(real);
+  if (p->fwd_bands > 1) {
+    if (p->fwd_chunk > 1)
+      launch_fwd_banded<real, K, CH>(p, st, jobs, grid, block);
+    else
+      launch_fwd_banded<real, K, 1>(p, st, jobs, grid, block);
+    return;
+  }
+  if (p->fwd_chunk > 1)
+    hipLaunchKernelGGL((batch_fwd_kernel<real, K, CH>), grid, block, 0, st, jobs, p->tiles[kForward]);
+  else
+    hipLaunchKernelGGL((batch_fwd_kernel<real, K, 1>), grid, block, 0, st, jobs, p->tiles[kForward]);
+}
+
+template <typename real, int RY, int CA>
+static void launch_back_staged(const gt_batch_plan *p, hipStream_t st, const Job<real> *jobs) {
+  const dim3 grid(p->n_tiles[kBackward]);
+  if (p->back_tile == 32)
+    hipLaunchKernelGGL((batch_back_staged_kernel<real, RY, CA, 32>), grid, dim3(256), 0, st, jobs,
+                       p->tiles[kBackward]);
+  else
+    hipLaunchKernelGGL((batch_back_staged_kernel<real, RY, CA, 16>), grid, dim3(256), 0, st, jobs,
+                       p->tiles[kBackward]);
+}
+
+template <typename real, int RY>
+static void launch_back_ry(const gt_batch_plan *p, hipStream_t st, const Job<real> *jobs) {
+  if (p->back_stage == 16)
+    launch_back_staged<real, RY, 16>(p, st, jobs);
+  else if (p->back_stage == 8)
+    launch_back_staged<real, RY, 8>(p, st, jobs);
+  else
+    hipLaunchKernelGGL((batch_back_kernel<real, RY>), dim3(p->n_tiles[kBackward]), dim3(256), 0, st,
+                       jobs, p->tiles[kBackward]);
+}
+
 template <typename real>
 static void launch_fwd(const gt_batch_plan *p, hipStream_t st, const Job<real> *jobs) {
   if (p->n_tiles[kRelayout])
@@ -307,16 +454,16 @@ static void launch_fwd(const gt_batch_plan *p, hipStream_t st, const Job<real> *
   const dim3 grid(p->n_tiles[kForward]), block(64, 4);
   switch (p->fwd_group) {
     case 16:
-      hipLaunchKernelGGL((batch_fwd_kernel<real, 16>), grid, block, 0, st, jobs, p->tiles[kForward]);
+      launch_fwd_k<real, 16>(p, st, jobs, grid, block);
       break;
     case 8:
-      hipLaunchKernelGGL((batch_fwd_kernel<real, 8>), grid, block, 0, st, jobs, p->tiles[kForward]);
+      launch_fwd_k<real, 8>(p, st, jobs, grid, block);
       break;
     case 4:
-      hipLaunchKernelGGL((batch_fwd_kernel<real, 4>), grid, block, 0, st, jobs, p->tiles[kForward]);
+      launch_fwd_k<real, 4>(p, st, jobs, grid, block);
       break;
     case 2:
-      hipLaunchKernelGGL((batch_fwd_kernel<real, 2>), grid, block, 0, st, jobs, p->tiles[kForward]);
+      launch_fwd_k<real, 2>(p, st, jobs, grid, block);
       break;
     default:
       hipLaunchKernelGGL((batch_fwd_kernel<real, 1>), grid, block, 0, st, jobs, p->tiles[kForward]);
@@ -327,23 +474,18 @@ template <typename real>
 static void launch_back(const gt_batch_plan *p, hipStream_t st, const Job<real> *jobs) {
   hipLaunchKernelGGL(batch_stage_kernel<real>, dim3(p->n_tiles[kStage]), dim3(256), 0, st, jobs,
                      p->tiles[kStage]);
-  const dim3 grid(p->n_tiles[kBackward]);
   switch (p->back_rows) {
     case 8:
-      hipLaunchKernelGGL((batch_back_kernel<real, 8>), grid, dim3(256), 0, st, jobs,
-                         p->tiles[kBackward]);
+      launch_back_ry<real, 8>(p, st, jobs);
       break;
     case 4:
-      hipLaunchKernelGGL((batch_back_kernel<real, 4>), grid, dim3(256), 0, st, jobs,
-                         p->tiles[kBackward]);
+      launch_back_ry<real, 4>(p, st, jobs);
       break;
     case 2:
-      hipLaunchKernelGGL((batch_back_kernel<real, 2>), grid, dim3(256), 0, st, jobs,
-                         p->tiles[kBackward]);
+      launch_back_ry<real, 2>(p, st, jobs);
       break;
     default:
-      hipLaunchKernelGGL((batch_back_kernel<real, 1>), grid, dim3(256), 0, st, jobs,
-                         p->tiles[kBackward]);
+      launch_back_ry<real, 1>(p, st, jobs);
   }
 }
 
@@ -392,9 +534,12 @@ extern "C" int gt_batch_plan_create(gt_batch_plan **plan, int dtype, int n_items
     const int rc = check_item(items[i]);
     if (rc) return rc;
   }
-  const int K = tuning().fwd_group, RY = tuning().back_rows;
-  return dtype == GT_FLOAT64 ? plan_create<double>(plan, dtype, n_items, items, K, RY)
-                             : plan_create<float>(plan, dtype, n_items, items, K, RY);
+  const Tuning &t = tuning();
+  return dtype == GT_FLOAT64
+             ? plan_create<double>(plan, dtype, n_items, items, t.fwd_group, t.back_rows, t.fwd_bands,
+                                   t.fwd_chunk, t.back_stage, t.back_tile)
+             : plan_create<float>(plan, dtype, n_items, items, t.fwd_group, t.back_rows, t.fwd_bands,
+                                  t.fwd_chunk, t.back_stage, t.back_tile);
 }
 
 extern "C" int gt_batch_plan_destroy(gt_batch_plan *plan) {

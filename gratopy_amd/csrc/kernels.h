@@ -32,7 +32,8 @@ struct alignas(sizeof(real) * 8) AngleRec {
 // Everything one projection needs.  Forward jobs use sino/so/imgX/imgY (an
 // x-fastest and a y-fastest copy of the image: row strides rowX/rowY, slice
 // strides sliceX/sliceY; rows may be padded with zeros), backward jobs
-// img/io/S/tab.  Epilogues (data shares the sinogram's strides):
+// img/io/S/tab.  Batched jobs also name src, the x-fastest image the copies
+// are made from (and the TV dual step reads).  Epilogues (data shares the sinogram's strides):
 //   mode 0: store the projection;
 //   mode 1: forward stores value - data[k] (Landweber residual), backward
 //           stores img[k] + scale * value (Landweber update);
@@ -50,8 +51,9 @@ struct Job {
   const real *ofs, *sdpd, *geo;
   real *sino;
   Strides3 so;
-  const real *imgX, *imgY;
+  const real *imgX, *imgY, *src;
   int64_t rowX, rowY, sliceX, sliceY;
+  real *part;  // batched jobs: the slice's band partials of a row-banded forward
   real *img;
   Strides3 io;
   const real *S;
@@ -122,13 +124,13 @@ __device__ __forceinline__ void store_back(const Job<real> &J, int x, int y, int
 
 // Dual step at pixel (x, y) of a 2-D slice (total_variation.cl:54-93 and
 // the projection onto the unit ball, 150-167): v += sigma grad(ubar), then
-// v /= max(1, |v|).  ubar is read through imgX (x fastest), v has 4 entries
+// v /= max(1, |v|).  ubar is read through src (x fastest), v has 4 entries
 // per pixel at 4 * (x + Nx y).
 template <typename real>
 __device__ __forceinline__ void tv_dual_pixel(const Job<real> &J, real *__restrict__ v, float sigma,
                                               int x, int y) {
   const int64_t i = x + (int64_t)J.Nx * y;
-  const real *u = J.imgX;
+  const real *u = J.src;
   const real c = u[i];
   real g0 = -c, g1 = -c, g2 = -c;
   g0 = x < J.Nx - 1 ? g0 + u[i + 1] : (real)0;
@@ -896,6 +898,63 @@ __device__ __forceinline__ void staged_angle(const AngleRec<real> &c, const Stag
     } else {
       q += c.c[1];
     }
+  }
+}
+
+// LDS-staged backprojection of one tile: the block (256 threads) owns the TX x
+// (256/TX)*RY pixels starting at (bx*TX, by*(256/TX)*RY) of slice z; per chunk
+// of CA angles it computes each angle's window (stage_window), copies it into
+// `rows` (CA * kStageWidth elements of LDS; zeros outside the padded sinogram),
+// then every lane adds the chunk's angles to its RY pixels (staged_angle).
+// Lanes outside the image take part in the copies and barriers only.
+template <typename real, bool FAN, int RY, int CA, int TX>
+__device__ __forceinline__ void back_staged_tile(const Job<real> &J, int bx, int by, int z,
+                                                 real *rows, StageWindow<real> *win) {
+  constexpr int TY = 256 / TX;  // lane rows
+  const int tid = threadIdx.x;
+  const int x = bx * TX + (tid % TX);
+  const int y0 = (by * TY + tid / TX) * RY;
+  const bool active = x < J.Nx && y0 < J.Ny;
+  const int xa = bx * TX, xb = min(xa + TX - 1, J.Nx - 1);
+  // rows evaluated: a lane whose first row is inside the image evaluates all RY
+  const int ya = by * TY * RY, yb = min(ya + TY * RY - 1, J.Ny + RY - 2);
+  const int Nsp = J.Ns + 4;
+  const real *Sz = J.S + (int64_t)z * J.Na * Nsp;
+  const real xr = (real)x, yr = (real)y0;
+  real acc[RY];
+#pragma unroll
+  for (int r = 0; r < RY; ++r) acc[r] = 0;
+  for (int a0 = 0; a0 < J.Na; a0 += CA) {
+    const int na = min(CA, J.Na - a0);
+    if (tid < na) win[tid] = stage_window<real, FAN>(J, a0 + tid, xa, xb, ya, yb);
+    __syncthreads();
+    for (int k = tid >> 6; k < na; k += 4) {
+      const int n = win[k].n, w0 = win[k].w0;
+      const real *src = Sz + (int64_t)(a0 + k) * Nsp;
+      for (int j = tid & 63; j < n; j += 64) {
+        const int g = w0 + j;
+        rows[k * kStageWidth + j] = (g >= 0 && g < Nsp) ? src[g] : (real)0;
+      }
+    }
+    __syncthreads();
+    if (active) {
+      for (int k = 0; k < na; ++k) {
+        const StageWindow<real> W = win[k];
+        const AngleRec<real> &c = J.tab[a0 + k];
+        if (W.n)
+          staged_angle<real, FAN, RY, true>(c, W, rows + k * kStageWidth, xr, yr, acc);
+        else
+          staged_angle<real, FAN, RY, false>(c, W, Sz + (int64_t)(a0 + k) * Nsp, xr, yr, acc);
+      }
+    }
+    __syncthreads();
+  }
+  if (!active) return;
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    const int y = y0 + r;
+    if (y >= J.Ny) break;
+    store_back(J, x, y, z, acc[r]);
   }
 }
 

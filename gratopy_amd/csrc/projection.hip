@@ -122,63 +122,12 @@ __global__ __launch_bounds__(256) void back_band_kernel(Job<real> J, real *part,
   back_pixels<real, FAN, RY, BUF>(J, x, y0, blockIdx.z, a0, a1, part, band);
 }
 
-// LDS-staged form (stage_window / staged_angle): the block's tile is TX x
-// (256/TX)*RY pixels (TX lanes along x, RY pixels per lane along y); per chunk
-// of CA angles the block computes each angle's window, copies it into LDS
-// (zeros outside the padded sinogram), then every lane adds the chunk's angles
-// to its RY pixels.  Lanes outside the image take part in the copies and
-// barriers only.
+// LDS-staged form (back_staged_tile): one tile of TX x (256/TX)*RY pixels per block.
 template <typename real, bool FAN, int RY, int CA, int TX>
 __global__ __launch_bounds__(256) void back_staged_kernel(Job<real> J) {
-  constexpr int TY = 256 / TX;  // lane rows
   __shared__ real rows[CA * kStageWidth];
   __shared__ StageWindow<real> win[CA];
-  const int tid = threadIdx.x;
-  const int x = blockIdx.x * TX + (tid % TX);
-  const int y0 = (blockIdx.y * TY + tid / TX) * RY;
-  const int z = blockIdx.z;
-  const bool active = x < J.Nx && y0 < J.Ny;
-  const int xa = blockIdx.x * TX, xb = min(xa + TX - 1, J.Nx - 1);
-  // rows evaluated: a lane whose first row is inside the image evaluates all RY
-  const int ya = blockIdx.y * TY * RY, yb = min(ya + TY * RY - 1, J.Ny + RY - 2);
-  const int Nsp = J.Ns + 4;
-  const real *Sz = J.S + (int64_t)z * J.Na * Nsp;
-  const real xr = (real)x, yr = (real)y0;
-  real acc[RY];
-#pragma unroll
-  for (int r = 0; r < RY; ++r) acc[r] = 0;
-  for (int a0 = 0; a0 < J.Na; a0 += CA) {
-    const int na = min(CA, J.Na - a0);
-    if (tid < na) win[tid] = stage_window<real, FAN>(J, a0 + tid, xa, xb, ya, yb);
-    __syncthreads();
-    for (int k = tid >> 6; k < na; k += 4) {
-      const int n = win[k].n, w0 = win[k].w0;
-      const real *src = Sz + (int64_t)(a0 + k) * Nsp;
-      for (int j = tid & 63; j < n; j += 64) {
-        const int g = w0 + j;
-        rows[k * kStageWidth + j] = (g >= 0 && g < Nsp) ? src[g] : (real)0;
-      }
-    }
-    __syncthreads();
-    if (active) {
-      for (int k = 0; k < na; ++k) {
-        const StageWindow<real> W = win[k];
-        const AngleRec<real> &c = J.tab[a0 + k];
-        if (W.n)
-          staged_angle<real, FAN, RY, true>(c, W, rows + k * kStageWidth, xr, yr, acc);
-        else
-          staged_angle<real, FAN, RY, false>(c, W, Sz + (int64_t)(a0 + k) * Nsp, xr, yr, acc);
-      }
-    }
-    __syncthreads();
-  }
-  if (!active) return;
-#pragma unroll
-  for (int r = 0; r < RY; ++r) {
-    const int y = y0 + r;
-    if (y >= J.Ny) break;
-    store_back(J, x, y, z, acc[r]);
-  }
+  back_staged_tile<real, FAN, RY, CA, TX>(J, blockIdx.x, blockIdx.y, blockIdx.z, rows, win);
 }
 
 template <typename real, bool FAN, int RY, int CA>

@@ -257,9 +257,11 @@ int gt_tv_adj_lambda(gt_stream_t stream, int dtype, void *lamb, void *fb, void *
  *     u_bar in img; odd steps swap the roles.  Start from zero img, img2,
  *     field and sino (lambda); after an even number of steps u is in img.
  * The plan keeps the items' pointers; results per item are bitwise those of
- * the single-problem entry points and elementwise TV kernels.  The forward
- * bin group and backprojection rows per lane are taken from the tuning knobs
- * (gt_tune_set) when the plan is created.
+ * the single-problem entry points and elementwise TV kernels under the same
+ * tuning knobs.  The forward bin group, row bands and chunked strip reads,
+ * the backprojection rows per lane and LDS staging (chunk, tile width) are
+ * taken from the tuning knobs (gt_tune_set) when the plan is created; angle
+ * bands of the backprojection do not apply.
  */
 typedef struct gt_batch_item {
   int geometry; /* GT_GEOMETRY_RADON or GT_GEOMETRY_FAN */

@@ -1,7 +1,9 @@
 """Batched uneven slices (gt_batch_*): every slice's result equals, bit for
 bit, the single-slice function with the same settings (same kernel bodies,
-same arithmetic), for projections, Landweber and TV, eager and graph-captured;
-and the single-slice path itself is pinned to the oracle elsewhere."""
+same arithmetic), for projections, Landweber and TV, eager and graph-captured,
+under plain launches and under chunked forward reads with the LDS-staged
+backprojection; the single-slice path itself is pinned to the oracle
+elsewhere."""
 
 from __future__ import annotations
 
@@ -16,6 +18,26 @@ from gratopy_amd.batch import (BatchPlan, batch_backprojection, batch_forwardpro
                                batch_landweber, batch_total_variation)
 
 pytestmark = pytest.mark.gpu
+
+# launch variants a plan can be built with (batches take no angle bands)
+BATCH_VARIANTS = {
+    "K4-RY1": dict(fwd_group=4, back_rows=1, fwd_chunk=1, back_stage=0, back_tile=16),
+    "K8xC4-RY4xS16T32": dict(fwd_group=8, back_rows=4, fwd_chunk=4, back_stage=16, back_tile=32),
+    "K16xC4-RY8xS8T16": dict(fwd_group=16, back_rows=8, fwd_chunk=4, back_stage=8, back_tile=16),
+    "K8xB8xC4-RY8xS16T32": dict(fwd_group=8, fwd_bands=8, back_rows=8, fwd_chunk=4, back_stage=16,
+                                back_tile=32),
+    "K4xB4-RY2": dict(fwd_group=4, fwd_bands=4, back_rows=2, fwd_chunk=1, back_stage=0, back_tile=16),
+}
+
+
+@pytest.fixture(params=list(BATCH_VARIANTS))
+def variant(request):
+    from gratopy_amd import autotune
+
+    before = autotune.current_variant()
+    autotune.set_variant(**{"fwd_bands": 1, "back_bands": 1, **BATCH_VARIANTS[request.param]})
+    yield request.param
+    autotune.set_variant(**before)
 
 
 def _mix(dtype, n=7, seed=0):
@@ -38,8 +60,8 @@ def _mix(dtype, n=7, seed=0):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_batch_projections_equal_single_slice(dtype):
-    print(f"batch projections {np.dtype(dtype).name}", flush=True)
+def test_batch_projections_equal_single_slice(dtype, variant):
+    print(f"batch projections {np.dtype(dtype).name} {variant}", flush=True)
     settings, imgs = _mix(dtype)
     sinos = batch_forwardprojection(imgs, settings)
     backs = batch_backprojection(sinos, settings)
@@ -52,8 +74,8 @@ def test_batch_projections_equal_single_slice(dtype):
 
 
 @pytest.mark.parametrize("graph", [False, True])
-def test_batch_landweber_equals_single_slice(graph):
-    print(f"batch landweber graph={graph}", flush=True)
+def test_batch_landweber_equals_single_slice(graph, variant):
+    print(f"batch landweber graph={graph} {variant}", flush=True)
     settings, imgs = _mix(np.float32, seed=1)
     sinos = [gratopy.forwardprojection(img, ps) for ps, img in zip(settings, imgs)]
     norms = [gratopy.normest(ps) for ps in settings]
@@ -65,8 +87,8 @@ def test_batch_landweber_equals_single_slice(graph):
 
 @pytest.mark.parametrize("n_iter", [1, 6, 7])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_batch_tv_equals_single_slice(n_iter, dtype):
-    print(f"batch tv {n_iter} {np.dtype(dtype).name}", flush=True)
+def test_batch_tv_equals_single_slice(n_iter, dtype, variant):
+    print(f"batch tv {n_iter} {np.dtype(dtype).name} {variant}", flush=True)
     settings, imgs = _mix(dtype, seed=2)
     sinos = [gratopy.forwardprojection(img, ps) for ps, img in zip(settings, imgs)]
     norms = [gratopy.normest(ps) for ps in settings]
