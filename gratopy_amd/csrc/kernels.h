@@ -748,10 +748,14 @@ __device__ __forceinline__ void back_pixels(const Job<real> &J, int x, int y0, i
   for (int a = a0; a < a1; ++a) {
     const AngleRec<real> &c = tab[a];
     const real *row = Sz + (int64_t)a * Nsp;
-    real t = FAN ? fma(c.c[0], xr, fma(c.c[1], yr, c.c[2])) : (real)1;
-    real q = FAN ? fma(c.c[3], xr, fma(c.c[4], yr, c.c[5])) : fma(c.c[0], xr, fma(c.c[1], yr, c.c[2]));
+    // row r of the lane: one FMA from row 0 (no running sum, which would add
+    // a rounding of a value ~Ns in size per row)
+    const real t0 = FAN ? fma(c.c[0], xr, fma(c.c[1], yr, c.c[2])) : (real)1;
+    const real q0 = FAN ? fma(c.c[3], xr, fma(c.c[4], yr, c.c[5])) : fma(c.c[0], xr, fma(c.c[1], yr, c.c[2]));
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
+      const real t = FAN ? fma(c.c[1], (real)r, t0) : (real)1;
+      const real q = fma(FAN ? c.c[4] : c.c[1], (real)r, q0);
       real p, w = 1;
       if (FAN) {
         w = A::rcp(t);
@@ -774,12 +778,6 @@ __device__ __forceinline__ void back_pixels(const Job<real> &J, int x, int y0, i
       }
       const real v = v0 + f * (v1 - v0);
       acc[r] = FAN ? acc[r] + v * w : acc[r] + v;
-      if (FAN) {
-        t += c.c[1];
-        q += c.c[4];
-      } else {
-        q += c.c[1];
-      }
     }
   }
 #pragma unroll
@@ -875,10 +873,13 @@ template <typename real, bool FAN, int RY, bool STAGED>
 __device__ __forceinline__ void staged_angle(const AngleRec<real> &c, const StageWindow<real> &W,
                                              const real *row, real xr, real yr, real *acc) {
   using A = Arith<real>;
-  real t = FAN ? fma(c.c[0], xr, fma(c.c[1], yr, W.k0)) : (real)1;
-  real q = FAN ? fma(c.c[3], xr, fma(c.c[4], yr, W.k1)) : fma(c.c[0], xr, fma(c.c[1], yr, W.k0));
+  const real t0 = FAN ? fma(c.c[0], xr, fma(c.c[1], yr, W.k0)) : (real)1;
+  const real q0 = FAN ? fma(c.c[3], xr, fma(c.c[4], yr, W.k1)) : fma(c.c[0], xr, fma(c.c[1], yr, W.k0));
 #pragma unroll
   for (int r = 0; r < RY; ++r) {
+    // row r of the lane: one FMA from row 0, as in back_pixels
+    const real t = FAN ? fma(c.c[1], (real)r, t0) : (real)1;
+    const real q = fma(FAN ? c.c[4] : c.c[1], (real)r, q0);
     real p, w = 1;
     if (FAN) {
       w = A::rcp(t);
@@ -892,12 +893,6 @@ __device__ __forceinline__ void staged_angle(const AngleRec<real> &c, const Stag
     const real v0 = tap[0], v1 = tap[1];
     const real v = v0 + f * (v1 - v0);
     acc[r] = FAN ? acc[r] + v * w : acc[r] + v;
-    if (FAN) {
-      t += c.c[1];
-      q += c.c[4];
-    } else {
-      q += c.c[1];
-    }
   }
 }
 
