@@ -53,6 +53,9 @@ def parse():
                    help="forward strip reads: 1 pixel or 4 (16 bytes) per load")
     p.add_argument("--no-batched", action="store_true",
                    help="skip the batched-vs-plain solver measurements")
+    p.add_argument("--uneven-only", action="store_true",
+                   help="only the batched-vs-plain Landweber and TV measurements on the 64 uneven "
+                        "slices (with launch-variant flags: those variants, no batch autotuning)")
     p.add_argument("--tv-iterations", type=int, default=6,
                    help="TV iterations timed per path on the 2048^2 stack (rate extrapolated to 500)")
     p.add_argument("--calibrate-copy", action="store_true",
@@ -151,10 +154,10 @@ def uneven_mix(n=64, seed=3):
     return out
 
 
-def uneven_problem():
+def uneven_problem(tune=True):
     """The 64-slice uneven mix with noisy sinograms, norm estimates, the
-    pixel-angle work of one fwd+back over all slices, and the batch
-    autotuner's choice (installed)."""
+    pixel-angle work of one fwd+back over all slices, and the launch variant
+    (the batch autotuner's choice, installed, unless tune is False)."""
     import gratopy_amd as gratopy
     from gratopy_amd import _device as dev
     from gratopy_amd import autotune
@@ -168,10 +171,12 @@ def uneven_problem():
         sinos.append(dev.to_device(np.asfortranarray(add_noise(s, 0.5, rng)), "cuda:0"))
     norms = [gratopy.normest(ps) for ps in settings]
     work = sum(ps.img_shape[0] * ps.img_shape[1] * ps.sinogram_shape[1] for ps in settings)
+    if not tune:
+        return settings, sinos, norms, work, {**autotune.current_variant(), "source": "fixed"}
     scratch_i = [dev.empty(ps.img_shape, np.float32, order="F", device="cuda:0") for ps in settings]
     scratch_s = [dev.empty(ps.sinogram_shape, np.float32, order="F", device="cuda:0") for ps in settings]
     tuned = autotune.tune_batch(settings, scratch_i, scratch_s)
-    return settings, sinos, norms, work, tuned
+    return settings, sinos, norms, work, {**tuned, "source": "autotuner"}
 
 
 def bench_uneven_landweber(problem, n_iter=100):
@@ -196,8 +201,8 @@ def bench_uneven_landweber(problem, n_iter=100):
             "batched_graph": round(n / tb, 3), "plain_per_slice": round(n / tp, 3),
             "speedup": round(tp / tb, 3), "fwd+back GUPS batched": round(work * n_iter / tb / 1e9, 2),
             "bitwise_equal": same,
-            "tuning": {**{k: tuned[k] for k in ("fwd_group", "fwd_bands", "fwd_chunk", "back_rows",
-                                                "back_stage", "back_tile")}, "source": "autotuner"}}
+            "tuning": {k: tuned[k] for k in ("fwd_group", "fwd_bands", "fwd_chunk", "back_rows",
+                                             "back_stage", "back_tile", "source")}}
 
 
 def bench_uneven_tv(problem, n_iter=100, mu=1e-2):
@@ -306,6 +311,14 @@ def main():
         _hip.check(lib.gt_tune_set(5, args.back_tile), "gt_tune_set")
     if args.fwd_chunk is not None:
         _hip.check(lib.gt_tune_set(6, args.fwd_chunk), "gt_tune_set")
+    fixed = any(v is not None for v in (args.fwd_group, args.back_rows, args.fwd_bands, args.back_bands,
+                                        args.back_stage, args.back_tile, args.fwd_chunk))
+    if args.uneven_only:
+        problem = uneven_problem(tune=not fixed)
+        out = {"batched": {"landweber_uneven": bench_uneven_landweber(problem),
+                           "tv_uneven": bench_uneven_tv(problem)}}
+        print(json.dumps(out), file=json_out, flush=True)
+        return
     Nz, N, Na, Ns, RE, R, iw, dw, n_ell = WORKLOADS[args.workload]
     ps_kwargs = dict(R=R, RE=RE, image_width=iw, detector_width=dw)
     device = "cuda:0"
@@ -318,8 +331,7 @@ def main():
     stream = torch.cuda.current_stream()
     tuning_source = "fixed"
     tuned = None
-    if all(v is None for v in (args.fwd_group, args.back_rows, args.fwd_bands, args.back_bands,
-                               args.back_stage, args.back_tile, args.fwd_chunk)):
+    if not fixed:
         from gratopy_amd import autotune
 
         back.copy_(img)
