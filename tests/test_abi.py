@@ -80,6 +80,34 @@ def test_workspace_sizes(lib):
     assert lib.gt_projection_workspace_bytes(1, 1, 1, 100, 60, 80, 30, 2, 0) >= 8 * (84 * 30 * 2 + 8 * 30)
 
 
+def test_tuning_knobs_accept_their_values_and_reject_others(lib):
+    from gratopy_amd import autotune
+
+    before = autotune.current_variant()
+    valid = {"fwd_group": (1, 2, 4, 8, 16), "back_rows": (1, 2, 4, 8), "fwd_bands": (1, 2, 4, 8),
+             "back_bands": (1, 2, 4, 8), "back_stage": (0, 8, 16), "back_tile": (16, 32),
+             "fwd_chunk": (1, 4)}
+    try:
+        assert set(valid) == set(autotune.KNOBS)
+        for name, values in valid.items():
+            key = autotune.KNOBS[name]
+            for v in values:
+                assert lib.gt_tune_set(key, v) == _hip.GT_OK and lib.gt_tune_get(key) == v
+            for v in (-1, 3, 5, 64):
+                if v not in values:
+                    assert lib.gt_tune_set(key, v) == _hip.GT_EINVAL, (name, v)
+                    assert lib.gt_tune_get(key) == values[-1]
+        assert lib.gt_tune_set(99, 1) == _hip.GT_EINVAL and lib.gt_tune_get(99) == -1
+        # the chunked forward needs two row-padded copies in its workspace
+        autotune.set_variant(fwd_group=8, fwd_bands=1, fwd_chunk=4)
+        padded = lib.gt_projection_workspace_bytes(1, 0, 0, 100, 60, 80, 30, 1, 0)
+        assert padded >= 4 * ((100 + 4) * 60 + (60 + 4) * 100)
+        autotune.set_variant(fwd_chunk=1)
+        assert lib.gt_projection_workspace_bytes(1, 0, 0, 100, 60, 80, 30, 1, 0) < padded
+    finally:
+        autotune.set_variant(**before)
+
+
 def test_batch_item_layout_matches_the_header(tmp_path):
     """ctypes' gt_batch_item has the C struct's size and field offsets (gcc on
     the header)."""
