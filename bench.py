@@ -51,6 +51,8 @@ def parse():
                    help="tile width (16 or 32 pixels) of the staged backprojection")
     p.add_argument("--fwd-chunk", type=int, default=None,
                    help="forward strip reads: 1 pixel or 4 (16 bytes) per load")
+    p.add_argument("--auto", action="store_true",
+                   help="no autotuning: the library's automatic launch shapes (its default)")
     p.add_argument("--no-batched", action="store_true",
                    help="skip the batched-vs-plain solver measurements")
     p.add_argument("--uneven-only", action="store_true",
@@ -297,22 +299,13 @@ def main():
 
     hip_build.ensure_built()
     lib = _hip.load_library()
-    if args.fwd_group is not None:
-        _hip.check(lib.gt_tune_set(0, args.fwd_group), "gt_tune_set")
-    if args.back_rows is not None:
-        _hip.check(lib.gt_tune_set(1, args.back_rows), "gt_tune_set")
-    if args.fwd_bands is not None:
-        _hip.check(lib.gt_tune_set(2, args.fwd_bands), "gt_tune_set")
-    if args.back_bands is not None:
-        _hip.check(lib.gt_tune_set(3, args.back_bands), "gt_tune_set")
-    if args.back_stage is not None:
-        _hip.check(lib.gt_tune_set(4, args.back_stage), "gt_tune_set")
-    if args.back_tile is not None:
-        _hip.check(lib.gt_tune_set(5, args.back_tile), "gt_tune_set")
-    if args.fwd_chunk is not None:
-        _hip.check(lib.gt_tune_set(6, args.fwd_chunk), "gt_tune_set")
-    fixed = any(v is not None for v in (args.fwd_group, args.back_rows, args.fwd_bands, args.back_bands,
-                                        args.back_stage, args.back_tile, args.fwd_chunk))
+    from gratopy_amd import autotune
+
+    flags = {k: getattr(args, k) for k in ("fwd_group", "back_rows", "fwd_bands", "back_bands",
+                                           "back_stage", "back_tile", "fwd_chunk")}
+    fixed = any(v is not None for v in flags.values())
+    if fixed:  # the given knobs; the others keep their values; no automatic choice
+        autotune.set_variant(**flags)
     if args.uneven_only:
         problem = uneven_problem(tune=not fixed)
         out = {"batched": {"landweber_uneven": bench_uneven_landweber(problem),
@@ -331,9 +324,9 @@ def main():
     stream = torch.cuda.current_stream()
     tuning_source = "fixed"
     tuned = None
-    if not fixed:
-        from gratopy_amd import autotune
-
+    if args.auto and not fixed:
+        tuning_source = "automatic"
+    elif not fixed:
         back.copy_(img)
         tuned = autotune.tune(ps, back, sino, repeats=2, store=False)
         tuning_source = "autotuner"
@@ -399,7 +392,7 @@ def main():
         "tuning": {"fwd_group": lib.gt_tune_get(0), "fwd_bands": lib.gt_tune_get(2),
                    "back_rows": lib.gt_tune_get(1), "back_bands": lib.gt_tune_get(3),
                    "back_stage": lib.gt_tune_get(4), "back_tile": lib.gt_tune_get(5),
-                   "fwd_chunk": lib.gt_tune_get(6),
+                   "fwd_chunk": lib.gt_tune_get(6), "automatic": lib.gt_tune_get(7),
                    "source": tuning_source,
                    "trial_ms": None if tuned is None else {"fwd": tuned["fwd_ms"], "back": tuned["back_ms"]}},
         "fwd_ms": round(fwd_med * 1e3, 3),

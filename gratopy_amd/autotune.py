@@ -1,5 +1,6 @@
 """Launch-shape autotuner for the projection kernels.
 
+By default every call picks its launch shape from its size (``automatic``).
 Knobs (the C ABI's ``GT_TUNE_*`` keys): the forward kernels assign K
 adjacent detector bins to a lane (``fwd_group`` K in 1, 2, 4, 8, 16; K = 1 is the
 per-ray kernel), optionally split the strip walk into 8 row bands
@@ -29,12 +30,13 @@ import numpy as np
 from . import _device as dev
 from . import _hip
 
-# knob name -> GT_TUNE_* key of the C ABI
+# knob name -> GT_TUNE_* key of the C ABI; "automatic" (GT_TUNE_AUTO) = 1 lets
+# every call pick its launch shape from its size and ignore the other knobs
 KNOBS = {"fwd_group": 0, "back_rows": 1, "fwd_bands": 2, "back_bands": 3, "back_stage": 4,
-         "back_tile": 5, "fwd_chunk": 6}
+         "back_tile": 5, "fwd_chunk": 6, "automatic": 7}
 # the library's values at load time
 DEFAULTS = {"fwd_group": 4, "back_rows": 1, "fwd_bands": 1, "back_bands": 1, "back_stage": 0,
-            "back_tile": 16, "fwd_chunk": 1}
+            "back_tile": 16, "fwd_chunk": 1, "automatic": 1}
 
 FWD_GROUPS = (1, 2, 4, 8, 16)
 BACK_ROWS = (1, 2, 4, 8)
@@ -91,13 +93,17 @@ def _store(table: dict) -> None:
 
 
 def set_variant(**knobs) -> None:
-    """Set launch-shape knobs (names of :data:`KNOBS`; None leaves a knob)."""
+    """Set launch-shape knobs (names of :data:`KNOBS`; None leaves a knob).
+    Setting any knob switches the automatic choice off unless ``automatic``
+    is given as well."""
     lib = _hip.load_library()
     for name, value in knobs.items():
         if name not in KNOBS:
             raise ValueError(f"unknown tuning knob {name!r}")
         if value is not None:
             _hip.check(lib.gt_tune_set(KNOBS[name], int(value)), "gt_tune_set")
+    if "automatic" not in knobs and any(v is not None for v in knobs.values()):
+        _hip.check(lib.gt_tune_set(KNOBS["automatic"], 0), "gt_tune_set")
 
 
 def current_variant() -> dict:
@@ -207,5 +213,5 @@ def apply(projectionsetting, dtype, nz: int = 1) -> bool:
     entry = _load().get(shape_key(projectionsetting, dtype, nz))
     if not entry:
         return False
-    set_variant(**{name: entry.get(name, DEFAULTS[name]) for name in KNOBS})
+    set_variant(**{name: entry.get(name, DEFAULTS[name]) for name in KNOBS if name != "automatic"})
     return True

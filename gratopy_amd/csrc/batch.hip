@@ -534,7 +534,17 @@ extern "C" int gt_batch_plan_create(gt_batch_plan **plan, int dtype, int n_items
     const int rc = check_item(items[i]);
     if (rc) return rc;
   }
-  const Tuning &t = tuning();
+  // automatic launch shapes: the batch autotuner's choice on 64 uneven slices
+  // (128^2..1024^2, 60..720 angles); batches have no backprojection angle bands
+  Tuning t = tuning();
+  if (t.automatic) {
+    t.fwd_group = 8;
+    t.fwd_bands = 8;
+    t.fwd_chunk = 4;
+    t.back_rows = 8;
+    t.back_stage = 16;
+    t.back_tile = 32;
+  }
   return dtype == GT_FLOAT64
              ? plan_create<double>(plan, dtype, n_items, items, t.fwd_group, t.back_rows, t.fwd_bands,
                                    t.fwd_chunk, t.back_stage, t.back_tile)

@@ -150,10 +150,10 @@ __global__ __launch_bounds__(256) void back_reduce_kernel(Job<real> J, const rea
 }
 
 template <typename real, bool FAN, int RY>
-static void launch_back(hipStream_t st, const Job<real> &J, int nbands, real *part) {
+static void launch_back(hipStream_t st, const Job<real> &J, int nbands, int stage, int tx,
+                        real *part) {
   const dim3 grid(((J.Nx + 15) / 16) * nbands, (J.Ny + 16 * RY - 1) / (16 * RY), J.Nz);
   const bool buf = buffer_fits(sizeof(real), J.Ns, J.Na);
-  const int stage = tuning().back_stage, tx = tuning().back_tile;
   if (nbands == 1 && stage == 8) {
     launch_staged<real, FAN, RY, 8>(st, J, tx);
   } else if (nbands == 1 && stage == 16) {
@@ -300,19 +300,49 @@ static int check_sizes(const Layout &L) {
 
 static size_t elem_size(int dtype) { return dtype == GT_FLOAT64 ? 8 : 4; }
 
-// row bands of the forward for the current tuning (bands need bin groups)
-static int fwd_bands() { return tuning().fwd_group > 1 ? tuning().fwd_bands : 1; }
+// Launch shape of one call: the tuning knobs, or (GT_TUNE_AUTO, the default)
+// a choice by the work of the call, Nx*Ny*Na*Nz pixel-angles, from autotuner
+// trials on MI355X (256^2 to 32 x 2048^2, parallel and fan beam): small
+// problems need row or angle bands to fill the GPU; large ones run the
+// chunked forward and the LDS-staged backprojection.
+struct FwdShape {
+  int K, bands, chunk;  // bands and chunked reads need bin groups (K > 1)
+};
+struct BackShape {
+  int RY, bands, stage, tile;  // staging applies with one band
+};
 
-// Chunked strip reads (tuning knob GT_TUNE_FWD_CHUNK = 4, bin groups only) read
-// 16 bytes per load and need both image copies with every row padded by
-// kRowPad zeros; otherwise an F-ordered input is read in place.
+static double work_of(const Layout &L) { return (double)L.Nx * L.Ny * L.Na * L.Nz; }
+
+static FwdShape fwd_shape(const Layout &L) {
+  const Tuning &t = tuning();
+  if (!t.automatic) {
+    const bool g = t.fwd_group > 1;
+    return {t.fwd_group, g ? t.fwd_bands : 1, g ? t.fwd_chunk : 1};
+  }
+  const double w = work_of(L);
+  if (w < 2e8) return {4, 8, 1};
+  if (w < 1.5e10) return {8, 8, 4};
+  return {16, 8, 4};
+}
+
+static BackShape back_shape(const Layout &L) {
+  const Tuning &t = tuning();
+  if (!t.automatic) return {t.back_rows, t.back_bands, t.back_bands > 1 ? 0 : t.back_stage, t.back_tile};
+  if (work_of(L) < 1e9) return {4, 8, 0, 16};
+  return {8, 1, 16, 32};
+}
+
+// Chunked strip reads (chunk = 4) read 16 bytes per load and need both image
+// copies with every row padded by kRowPad zeros; otherwise an F-ordered input
+// is read in place.
 constexpr int kRowPad = 4;
-static bool fwd_chunked() { return tuning().fwd_group > 1 && tuning().fwd_chunk > 1; }
 
 static size_t fwd_ws(int dtype, const Layout &L, int img_order) {
   const size_t e = elem_size(dtype);
+  const FwdShape sh = fwd_shape(L);
   size_t n = 0;
-  if (fwd_chunked()) {
+  if (sh.chunk > 1) {
     n += align_up(e * (size_t)(L.Nx + kRowPad) * L.Ny * L.Nz);  // x-fastest padded copy
     n += align_up(e * (size_t)(L.Ny + kRowPad) * L.Nx * L.Nz);  // y-fastest padded copy
   } else {
@@ -320,13 +350,12 @@ static size_t fwd_ws(int dtype, const Layout &L, int img_order) {
     if (img_order != GT_ORDER_F) n += img;                  // x-fastest copy
     if (!(img_order == GT_ORDER_C && L.Nz == 1)) n += img;  // y-fastest copy
   }
-  const int nb = fwd_bands();
-  if (nb > 1) n += align_up(e * (size_t)nb * L.Ns * L.Na * L.Nz);  // band partials
+  if (sh.bands > 1) n += align_up(e * (size_t)sh.bands * L.Ns * L.Na * L.Nz);  // band partials
   return n;
 }
 
 static size_t back_partials(int dtype, const Layout &L) {
-  const int nb = tuning().back_bands;
+  const int nb = back_shape(L).bands;
   return nb > 1 ? align_up(elem_size(dtype) * (size_t)nb * L.Nx * L.Ny * L.Nz) : 0;
 }
 
@@ -360,10 +389,10 @@ __global__ __launch_bounds__(256) void pad_rows_kernel(real *__restrict__ dst, i
 }
 
 template <typename real, bool FAN, int K>
-static void launch_fwd_group(hipStream_t st, const Job<real> &J, dim3 grid, int nb, real *part) {
+static void launch_fwd_group(hipStream_t st, const Job<real> &J, dim3 grid, int nb, bool chunk,
+                             real *part) {
   constexpr int CH = 16 / sizeof(real);
   const dim3 block(64, 4);
-  const bool chunk = fwd_chunked();
   if (nb > 1) {
     if (chunk)
       hipLaunchKernelGGL((fwd_band_kernel<real, FAN, K, CH>), grid, block, 0, st, J, part, nb);
@@ -391,7 +420,8 @@ static int fwd_impl(hipStream_t st, real *sino, const real *img, const real *ofs
   J.rowX = L.Nx;
   J.rowY = L.Ny;
   J.sliceX = J.sliceY = (int64_t)L.Nx * L.Ny;
-  if (fwd_chunked()) {
+  const FwdShape sh = fwd_shape(L);
+  if (sh.chunk > 1) {
     const int64_t rx = L.Nx + kRowPad, ry = L.Ny + kRowPad;
     real *cx = reinterpret_cast<real *>(w);
     w += align_up(sizeof(real) * (size_t)rx * L.Ny * L.Nz);
@@ -427,9 +457,9 @@ static int fwd_impl(hipStream_t st, real *sino, const real *img, const real *ofs
   }
   J.sino = sino;
   J.so = order_strides(sino_order, L.Ns, L.Na, L.Nz);
-  const int K = tuning().fwd_group;
+  const int K = sh.K, nb = sh.bands;
+  const bool chunk = sh.chunk > 1;
   const int groups = (L.Ns + 64 * K - 1) / (64 * K);
-  const int nb = fwd_bands();
   real *part = nullptr;
   if (nb > 1) {
     GT_REQUIRE((int64_t)groups * nb < ((int64_t)1 << 31), "too many detector groups");
@@ -439,13 +469,13 @@ static int fwd_impl(hipStream_t st, real *sino, const real *img, const real *ofs
   }
   const dim3 grid(groups * nb, (L.Na + 3) / 4, L.Nz);
   if (K == 16)
-    launch_fwd_group<real, FAN, 16>(st, J, grid, nb, part);
+    launch_fwd_group<real, FAN, 16>(st, J, grid, nb, chunk, part);
   else if (K == 8)
-    launch_fwd_group<real, FAN, 8>(st, J, grid, nb, part);
+    launch_fwd_group<real, FAN, 8>(st, J, grid, nb, chunk, part);
   else if (K == 4)
-    launch_fwd_group<real, FAN, 4>(st, J, grid, nb, part);
+    launch_fwd_group<real, FAN, 4>(st, J, grid, nb, chunk, part);
   else if (K == 2)
-    launch_fwd_group<real, FAN, 2>(st, J, grid, nb, part);
+    launch_fwd_group<real, FAN, 2>(st, J, grid, nb, chunk, part);
   else
     hipLaunchKernelGGL((fwd_kernel<real, FAN, 1>), grid, dim3(64, 4), 0, st, J);
   if (nb > 1)
@@ -474,20 +504,20 @@ static int back_impl(hipStream_t st, real *img, const real *sino, const real *of
   J.io = order_strides(img_order, L.Nx, L.Ny, L.Nz);
   J.S = S;
   J.tab = tab;
-  const int RY = tuning().back_rows;
-  const int nb = tuning().back_bands;
+  const BackShape sh = back_shape(L);
+  const int RY = sh.RY, nb = sh.bands;
   GT_REQUIRE((int64_t)((L.Nx + 15) / 16) * nb < ((int64_t)1 << 31), "image too wide");
   // the band partials are the last part of the workspace
   const size_t part_bytes = back_partials(sizeof(real) == 8 ? GT_FLOAT64 : GT_FLOAT32, L);
   real *part = reinterpret_cast<real *>(w + (need - part_bytes));
   if (RY == 8)
-    launch_back<real, FAN, 8>(st, J, nb, part);
+    launch_back<real, FAN, 8>(st, J, nb, sh.stage, sh.tile, part);
   else if (RY == 4)
-    launch_back<real, FAN, 4>(st, J, nb, part);
+    launch_back<real, FAN, 4>(st, J, nb, sh.stage, sh.tile, part);
   else if (RY == 2)
-    launch_back<real, FAN, 2>(st, J, nb, part);
+    launch_back<real, FAN, 2>(st, J, nb, sh.stage, sh.tile, part);
   else
-    launch_back<real, FAN, 1>(st, J, nb, part);
+    launch_back<real, FAN, 1>(st, J, nb, sh.stage, sh.tile, part);
   return check_launch(FAN ? "fan backprojection" : "radon backprojection");
 }
 

@@ -59,6 +59,10 @@ extern "C" int gt_tune_set(int key, int value) {
         return gt::fail(GT_EINVAL, "backprojection staging chunk must be 0, 8 or 16 angles");
       t.back_stage = value;
       return GT_OK;
+    case GT_TUNE_AUTO:
+      if (value != 0 && value != 1) return gt::fail(GT_EINVAL, "automatic launch shapes: 0 or 1");
+      t.automatic = value;
+      return GT_OK;
     case GT_TUNE_FWD_CHUNK:
       if (value != 1 && value != 4)
         return gt::fail(GT_EINVAL, "forward chunk must be 1 or 4");
@@ -91,6 +95,8 @@ extern "C" int gt_tune_get(int key) {
       return t.back_tile;
     case GT_TUNE_FWD_CHUNK:
       return t.fwd_chunk;
+    case GT_TUNE_AUTO:
+      return t.automatic;
     default:
       return -1;
   }

@@ -64,7 +64,10 @@ int gt_version(void);
  *   GT_TUNE_FWD_CHUNK: 4 = the forward bin groups read their strip rows 16 bytes per
  *                      load (4 float32 or 2 float64 pixels) from row-padded copies of
  *                      the image (the workspace then holds two padded copies); 1 = one
- *                      pixel per load. */
+ *                      pixel per load.
+ *   GT_TUNE_AUTO:      1 (the default) = every call (and batch plan) picks its launch
+ *                      shape from its size and the knobs above are ignored; 0 = the
+ *                      knobs above apply.  Workspace sizes follow the same choice. */
 enum {
   GT_TUNE_FWD_GROUP = 0,
   GT_TUNE_BACK_ROWS = 1,
@@ -72,7 +75,8 @@ enum {
   GT_TUNE_BACK_BANDS = 3,
   GT_TUNE_BACK_STAGE = 4,
   GT_TUNE_BACK_TILE = 5,
-  GT_TUNE_FWD_CHUNK = 6
+  GT_TUNE_FWD_CHUNK = 6,
+  GT_TUNE_AUTO = 7
 };
 int gt_tune_set(int key, int value);
 int gt_tune_get(int key);

@@ -86,7 +86,7 @@ def test_tuning_knobs_accept_their_values_and_reject_others(lib):
     before = autotune.current_variant()
     valid = {"fwd_group": (1, 2, 4, 8, 16), "back_rows": (1, 2, 4, 8), "fwd_bands": (1, 2, 4, 8),
              "back_bands": (1, 2, 4, 8), "back_stage": (0, 8, 16), "back_tile": (16, 32),
-             "fwd_chunk": (1, 4)}
+             "fwd_chunk": (1, 4), "automatic": (0, 1)}
     try:
         assert set(valid) == set(autotune.KNOBS)
         for name, values in valid.items():
@@ -100,6 +100,7 @@ def test_tuning_knobs_accept_their_values_and_reject_others(lib):
         assert lib.gt_tune_set(99, 1) == _hip.GT_EINVAL and lib.gt_tune_get(99) == -1
         # the chunked forward needs two row-padded copies in its workspace
         autotune.set_variant(fwd_group=8, fwd_bands=1, fwd_chunk=4)
+        assert lib.gt_tune_get(autotune.KNOBS["automatic"]) == 0
         padded = lib.gt_projection_workspace_bytes(1, 0, 0, 100, 60, 80, 30, 1, 0)
         assert padded >= 4 * ((100 + 4) * 60 + (60 + 4) * 100)
         autotune.set_variant(fwd_chunk=1)

@@ -156,11 +156,13 @@ def test_autotuner_picks_a_variant_and_caches_it(tmp_path, monkeypatch, restore_
     sino = gratopy.forwardprojection(img, ps)
     best = autotune.tune(ps, img.clone(), sino.clone(), repeats=2)
     assert best["fwd_group"] in autotune.FWD_GROUPS and best["back_rows"] in autotune.BACK_ROWS
-    assert autotune.current_variant() == {k: best[k] for k in autotune.KNOBS}
+    chosen = {k: best[k] for k in autotune.KNOBS if k != "automatic"}
+    assert autotune.current_variant() == {**chosen, "automatic": 0}
     assert set(best["fwd_ms"]) == set(FWD_IDS) and set(best["back_ms"]) == set(BACK_IDS)
     autotune.set_variant(**autotune.DEFAULTS)
+    assert autotune.current_variant()["automatic"] == 1
     assert autotune.apply(ps, F32, 1)
-    assert autotune.current_variant() == {k: best[k] for k in autotune.KNOBS}
+    assert autotune.current_variant() == {**chosen, "automatic": 0}
     # all variants agree to rounding on the tuned geometry
     ref = dev.to_host(gratopy.forwardprojection(img, ps))
     for variant in autotune.FWD_VARIANTS:
