@@ -22,6 +22,7 @@ from __future__ import annotations
 import ctypes
 import hashlib
 import struct
+import threading
 from dataclasses import dataclass
 from pathlib import Path
 from typing import Any
@@ -317,10 +318,11 @@ class _HipOperator(Operator):
         first use and cached by (device, source signature)."""
         device = dev.torch.device(queue.device)
         key = (str(device), self.kernel_spec.signature)
-        program = _PROGRAM_CACHE.get(key)
-        if program is None:
-            program = PluginProgram(self.kernel_spec, device)
-            _PROGRAM_CACHE[key] = program
+        with _PROGRAM_LOCK:
+            program = _PROGRAM_CACHE.get(key)
+            if program is None:
+                program = PluginProgram(self.kernel_spec, device)
+                _PROGRAM_CACHE[key] = program
         return program.kernels
 
     def _get_kernel(self, output, argument, queue: dev.Queue) -> int:
@@ -373,10 +375,12 @@ class _HipOperator(Operator):
 # may still launch their kernels.
 _PROGRAM_CACHE: dict[tuple[str, str], PluginProgram] = {}
 _RETIRED_PROGRAMS: list[PluginProgram] = []
+_PROGRAM_LOCK = threading.Lock()
 
 
 def invalidate_kernel_cache() -> None:
     """Forget every compiled plugin, so the next use compiles its sources
     again (opencl.py:457-459)."""
-    _RETIRED_PROGRAMS.extend(_PROGRAM_CACHE.values())
-    _PROGRAM_CACHE.clear()
+    with _PROGRAM_LOCK:
+        _RETIRED_PROGRAMS.extend(_PROGRAM_CACHE.values())
+        _PROGRAM_CACHE.clear()
