@@ -6,10 +6,10 @@ adjacent detector bins to a lane (``fwd_group`` K in 1, 2, 4, 8, 16; K = 1 is th
 per-ray kernel), optionally split the strip walk into 8 row bands
 (``fwd_bands``; one band per XCD, partial sums added in band order) and read
 strip rows 16 bytes per load (``fwd_chunk`` = 4).  The backprojection assigns
-RY pixels along y to a lane (``back_rows`` in 1, 2, 4, 8) and either sums all
-angles per tile, splits them into 8 angle bands (``back_bands``), or stages
-each tile's sinogram windows in LDS for chunks of 8 or 16 angles
-(``back_stage``) in tiles 16 or 32 pixels wide (``back_tile``).
+RY pixels along y to a lane (``back_rows`` in 1, 2, 4, 8), sums all angles per
+tile or splits them into 8 angle bands (``back_bands``), and reads its taps
+from memory or stages each tile's sinogram windows in LDS for chunks of 8 or
+16 angles (``back_stage``) in tiles 16 or 32 pixels wide (``back_tile``).
 Batched launches take all but the angle bands.  :func:`tune` times every variant on the
 caller's own geometry and array shapes with HIP events, keeps the fastest,
 and remembers it per shape key in a JSON cache
@@ -48,12 +48,15 @@ BACK_TILES = (16, 32)
 FWD_VARIANTS = ({"fwd_group": 1, "fwd_bands": 1, "fwd_chunk": 1},) + tuple(
     {"fwd_group": k, "fwd_bands": b, "fwd_chunk": c}
     for k in FWD_GROUPS if k > 1 for b in FWD_BANDS for c in FWD_CHUNKS)
-# backprojection variants: unstaged (with or without angle bands) and staged
+# backprojection variants: unstaged (with or without angle bands), staged, and
+# staged in chunks of 16 angles over 8 angle bands
 BACK_VARIANTS = tuple(
     {"back_rows": r, "back_bands": b, "back_stage": 0, "back_tile": 16}
     for b in (1, 8) for r in BACK_ROWS) + tuple(
     {"back_rows": r, "back_bands": 1, "back_stage": s, "back_tile": t}
-    for s in BACK_STAGES if s for t in BACK_TILES for r in BACK_ROWS)
+    for s in BACK_STAGES if s for t in BACK_TILES for r in BACK_ROWS) + tuple(
+    {"back_rows": r, "back_bands": 8, "back_stage": 16, "back_tile": t}
+    for t in BACK_TILES for r in BACK_ROWS)
 
 _lock = threading.Lock()
 
@@ -122,7 +125,7 @@ def variant_key(v: dict) -> str:
         return key
     key = f"RY{v['back_rows']}"
     if v.get("back_stage", 0):
-        return key + f"xS{v['back_stage']}T{v.get('back_tile', 16)}"
+        key += f"xS{v['back_stage']}T{v.get('back_tile', 16)}"
     if v.get("back_bands", 1) > 1:
         key += f"xB{v['back_bands']}"
     return key

@@ -45,6 +45,41 @@ namespace gt {
 enum TileKind { kRelayout = 0, kForward = 1, kStage = 2, kBackward = 3, kDual = 4 };
 enum JobKind { kPlain = 0, kLandweber = 1, kTvEven = 2, kTvOdd = 3 };
 
+// The empty asm hands back the (uniform) pointer typed as global; accesses
+// through it then compile to global and scalar loads.  A plain cast to global
+// and back would be folded away.
+template <typename T>
+__device__ __forceinline__ T *global_ptr(T *p) {
+  __attribute__((address_space(1))) T *g;
+  asm("" : "=s"(g) : "0"(p));
+  return (T *)g;
+}
+
+// A batched kernel's job comes from a table in global memory, so the compiler
+// cannot tell where its pointers point and would access them with generic
+// (flat) instructions, which also count against the LDS wait counter.  The
+// copy marks every pointer as global (all of them are device allocations), as
+// the pointers of a by-value kernel argument are.
+template <typename real>
+__device__ __forceinline__ Job<real> load_job(const Job<real> *__restrict__ jobs, int i) {
+  Job<real> J = jobs[i];
+  J.ofs = global_ptr(J.ofs);
+  J.sdpd = global_ptr(J.sdpd);
+  J.geo = global_ptr(J.geo);
+  J.sino = global_ptr(J.sino);
+  J.imgX = global_ptr(J.imgX);
+  J.imgY = global_ptr(J.imgY);
+  J.src = global_ptr(J.src);
+  J.part = global_ptr(J.part);
+  J.img = global_ptr(J.img);
+  J.S = global_ptr(J.S);
+  J.tab = global_ptr(J.tab);
+  J.data = global_ptr(J.data);
+  J.u = global_ptr(J.u);
+  J.v = global_ptr(J.v);
+  return J;
+}
+
 // tile (slice, 32x32 block): the forward's image copies of src (the y-fastest
 // one, and the row-padded x-fastest one when imgX is not src itself)
 template <typename real>
@@ -52,7 +87,7 @@ __global__ __launch_bounds__(256) void batch_relayout_kernel(const Job<real> *__
                                                              const int4 *__restrict__ tiles) {
   __shared__ real tile[32][33];
   const int4 t = tiles[blockIdx.x];
-  const Job<real> &J = jobs[t.x];
+  const Job<real> J = load_job(jobs, t.x);
   const Strides3 ss = {1, J.Nx, (int64_t)J.Nx * J.Ny};
   const Strides3 ys = {J.rowY, 1, J.sliceY};
   relayout_tile(tile, const_cast<real *>(J.imgY), ys, J.src, ss, J.Nx, J.Ny, 0, t.y, t.z);
@@ -66,7 +101,7 @@ template <typename real, int K, int CH = 1>
 __global__ __launch_bounds__(256) void batch_fwd_kernel(const Job<real> *__restrict__ jobs,
                                                         const int4 *__restrict__ tiles) {
   const int4 t = tiles[blockIdx.x];
-  const Job<real> &J = jobs[t.x];
+  const Job<real> J = load_job(jobs, t.x);
   const int a = t.z * 4 + threadIdx.y;
   if constexpr (K == 1) {
     const int s = t.y * 64 + threadIdx.x;
@@ -94,7 +129,7 @@ __global__ __launch_bounds__(256) void batch_fwd_band_kernel(const Job<real> *__
                                                              const int4 *__restrict__ tiles) {
   __shared__ real slots[Slots<K>::kCount * 256];
   const int4 t = tiles[blockIdx.x];
-  const Job<real> &J = jobs[t.x];
+  const Job<real> J = load_job(jobs, t.x);
   const int a = t.z * 4 + threadIdx.y;
   const int s0 = (t.y * 64 + threadIdx.x) * K;
   if (s0 >= J.Ns || a >= J.Na) return;
@@ -112,7 +147,7 @@ template <typename real, int NB>
 __global__ __launch_bounds__(256) void batch_band_reduce_kernel(const Job<real> *__restrict__ jobs,
                                                                 const int4 *__restrict__ tiles) {
   const int4 t = tiles[blockIdx.x];
-  const Job<real> &J = jobs[t.x];
+  const Job<real> J = load_job(jobs, t.x);
   for (int s = threadIdx.x; s < J.Ns; s += 256) band_reduce_entry(J, J.part, NB, s, t.y, 0);
 }
 
@@ -121,7 +156,7 @@ template <typename real>
 __global__ __launch_bounds__(256) void batch_stage_kernel(const Job<real> *__restrict__ jobs,
                                                           const int4 *__restrict__ tiles) {
   const int4 t = tiles[blockIdx.x];
-  const Job<real> &J = jobs[t.x];
+  const Job<real> J = load_job(jobs, t.x);
   for (int j = threadIdx.x; j < J.Ns + 4; j += 256)
     stage_entry(const_cast<real *>(J.S), J.sino, J.so, J.ofs, J.sdpd, J.geo, J.fan, J.Ns, J.Na, j,
                 t.y, 0);
@@ -131,7 +166,7 @@ template <typename real, int RY>
 __global__ __launch_bounds__(256) void batch_back_kernel(const Job<real> *__restrict__ jobs,
                                                          const int4 *__restrict__ tiles) {
   const int4 t = tiles[blockIdx.x];
-  const Job<real> &J = jobs[t.x];
+  const Job<real> J = load_job(jobs, t.x);
   const int x = t.y * 16 + (threadIdx.x & 15);
   const int y0 = (t.z * 16 + (threadIdx.x >> 4)) * RY;
   if (x >= J.Nx || y0 >= J.Ny) return;
@@ -148,7 +183,7 @@ __global__ __launch_bounds__(256) void batch_back_staged_kernel(const Job<real> 
   __shared__ real rows[CA * kStageWidth];
   __shared__ StageWindow<real> win[CA];
   const int4 t = tiles[blockIdx.x];
-  const Job<real> &J = jobs[t.x];
+  const Job<real> J = load_job(jobs, t.x);
   if (J.fan)
     back_staged_tile<real, true, RY, CA, TX>(J, t.y, t.z, 0, rows, win);
   else
@@ -161,7 +196,7 @@ __global__ __launch_bounds__(256) void batch_dual_kernel(const Job<real> *__rest
                                                          const int4 *__restrict__ tiles,
                                                          float sigma) {
   const int4 t = tiles[blockIdx.x];
-  const Job<real> &J = jobs[t.x];
+  const Job<real> J = load_job(jobs, t.x);
   const int64_t p = (int64_t)t.y * 256 + threadIdx.x;
   if (p >= (int64_t)J.Nx * J.Ny) return;
   tv_dual_pixel(J, const_cast<real *>(J.v), sigma, (int)(p % J.Nx), (int)(p / J.Nx));
@@ -171,7 +206,7 @@ template <typename real>
 __global__ void batch_table_kernel(const Job<real> *__restrict__ jobs, int n_items) {
   const int i = blockIdx.y;
   if (i >= n_items) return;
-  const Job<real> &J = jobs[i];
+  const Job<real> J = load_job(jobs, i);
   for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < J.Na; a += gridDim.x * blockDim.x)
     back_table_entry(const_cast<AngleRec<real> *>(J.tab), J.ofs, J.geo, J.fan, a);
 }

@@ -889,8 +889,20 @@ __device__ __forceinline__ void staged_angle(const AngleRec<real> &c, const Stag
     }
     if (!STAGED) p = A::clamp(p, W.lo, W.hi);
     const real f = A::fract(p);
-    const real *tap = row + (int)p;
-    const real v0 = tap[0], v1 = tap[1];
+    real v0, v1;
+    if (STAGED) {
+      // An LDS-qualified pointer: the compiler cannot merge these taps with the
+      // global reads of the direct path into one generic (flat) load, which
+      // would wait on global memory for every LDS tap.
+      const __attribute__((address_space(3))) real *tap =
+          (const __attribute__((address_space(3))) real *)row + (int)p;
+      v0 = tap[0];
+      v1 = tap[1];
+    } else {
+      const real *tap = row + (int)p;
+      v0 = tap[0];
+      v1 = tap[1];
+    }
     const real v = v0 + f * (v1 - v0);
     acc[r] = FAN ? acc[r] + v * w : acc[r] + v;
   }
@@ -901,10 +913,16 @@ __device__ __forceinline__ void staged_angle(const AngleRec<real> &c, const Stag
 // of CA angles it computes each angle's window (stage_window), copies it into
 // `rows` (CA * kStageWidth elements of LDS; zeros outside the padded sinogram),
 // then every lane adds the chunk's angles to its RY pixels (staged_angle).
-// Lanes outside the image take part in the copies and barriers only.
+// Lanes outside the image take part in the copies and barriers only.  Angles
+// [a_begin, a_end) only; with part != nullptr the raw sums go to
+// part[((band*Nz + z)*Ny + y)*Nx + x] (band_back_reduce_entry adds the bands in
+// order and applies the epilogue), otherwise they are stored.
 template <typename real, bool FAN, int RY, int CA, int TX>
 __device__ __forceinline__ void back_staged_tile(const Job<real> &J, int bx, int by, int z,
-                                                 real *rows, StageWindow<real> *win) {
+                                                 real *rows, StageWindow<real> *win, int a_begin = 0,
+                                                 int a_end = -1, real *part = nullptr,
+                                                 int band = 0) {
+  if (a_end < 0) a_end = J.Na;
   constexpr int TY = 256 / TX;  // lane rows
   const int tid = threadIdx.x;
   const int x = bx * TX + (tid % TX);
@@ -919,8 +937,8 @@ __device__ __forceinline__ void back_staged_tile(const Job<real> &J, int bx, int
   real acc[RY];
 #pragma unroll
   for (int r = 0; r < RY; ++r) acc[r] = 0;
-  for (int a0 = 0; a0 < J.Na; a0 += CA) {
-    const int na = min(CA, J.Na - a0);
+  for (int a0 = a_begin; a0 < a_end; a0 += CA) {
+    const int na = min(CA, a_end - a0);
     if (tid < na) win[tid] = stage_window<real, FAN>(J, a0 + tid, xa, xb, ya, yb);
     __syncthreads();
     for (int k = tid >> 6; k < na; k += 4) {
@@ -949,7 +967,10 @@ __device__ __forceinline__ void back_staged_tile(const Job<real> &J, int bx, int
   for (int r = 0; r < RY; ++r) {
     const int y = y0 + r;
     if (y >= J.Ny) break;
-    store_back(J, x, y, z, acc[r]);
+    if (part)
+      part[(((int64_t)band * J.Nz + z) * J.Ny + y) * J.Nx + x] = acc[r];
+    else
+      store_back(J, x, y, z, acc[r]);
   }
 }
 

@@ -122,25 +122,6 @@ __global__ __launch_bounds__(256) void back_band_kernel(Job<real> J, real *part,
   back_pixels<real, FAN, RY, BUF>(J, x, y0, blockIdx.z, a0, a1, part, band);
 }
 
-// LDS-staged form (back_staged_tile): one tile of TX x (256/TX)*RY pixels per block.
-template <typename real, bool FAN, int RY, int CA, int TX>
-__global__ __launch_bounds__(256) void back_staged_kernel(Job<real> J) {
-  __shared__ real rows[CA * kStageWidth];
-  __shared__ StageWindow<real> win[CA];
-  back_staged_tile<real, FAN, RY, CA, TX>(J, blockIdx.x, blockIdx.y, blockIdx.z, rows, win);
-}
-
-template <typename real, bool FAN, int RY, int CA>
-static void launch_staged(hipStream_t st, const Job<real> &J, int tx) {
-  if (tx == 32) {
-    const dim3 grid((J.Nx + 31) / 32, (J.Ny + 8 * RY - 1) / (8 * RY), J.Nz);
-    hipLaunchKernelGGL((back_staged_kernel<real, FAN, RY, CA, 32>), grid, dim3(256), 0, st, J);
-  } else {
-    const dim3 grid((J.Nx + 15) / 16, (J.Ny + 16 * RY - 1) / (16 * RY), J.Nz);
-    hipLaunchKernelGGL((back_staged_kernel<real, FAN, RY, CA, 16>), grid, dim3(256), 0, st, J);
-  }
-}
-
 template <typename real>
 __global__ __launch_bounds__(256) void back_reduce_kernel(Job<real> J, const real *part,
                                                           int nbands) {
@@ -149,15 +130,60 @@ __global__ __launch_bounds__(256) void back_reduce_kernel(Job<real> J, const rea
   if (x < J.Nx && y < J.Ny) band_back_reduce_entry(J, part, nbands, x, y, blockIdx.z);
 }
 
+// LDS-staged form (back_staged_tile): one tile of TX x (256/TX)*RY pixels per block.
+template <typename real, bool FAN, int RY, int CA, int TX>
+__global__ __launch_bounds__(256) void back_staged_kernel(Job<real> J) {
+  __shared__ real rows[CA * kStageWidth];
+  __shared__ StageWindow<real> win[CA];
+  back_staged_tile<real, FAN, RY, CA, TX>(J, blockIdx.x, blockIdx.y, blockIdx.z, rows, win);
+}
+
+// LDS-staged and angle-banded: blockIdx.x = band + nbands * tile column (as
+// back_band_kernel); each block stages and sums the angles of its band into
+// the band partials, which back_reduce_kernel adds in band order.
+template <typename real, bool FAN, int RY, int CA, int TX>
+__global__ __launch_bounds__(256) void back_staged_band_kernel(Job<real> J, real *part, int nbands) {
+  __shared__ real rows[CA * kStageWidth];
+  __shared__ StageWindow<real> win[CA];
+  const int band = blockIdx.x % nbands;
+  const int a0 = (int)((int64_t)J.Na * band / nbands);
+  const int a1 = (int)((int64_t)J.Na * (band + 1) / nbands);
+  back_staged_tile<real, FAN, RY, CA, TX>(J, blockIdx.x / nbands, blockIdx.y, blockIdx.z, rows, win,
+                                          a0, a1, part, band);
+}
+
+template <typename real, bool FAN, int RY, int CA, int TX>
+static void launch_staged_tx(hipStream_t st, const Job<real> &J, int nbands, real *part) {
+  const dim3 grid(((J.Nx + TX - 1) / TX) * nbands, (J.Ny + (256 / TX) * RY - 1) / ((256 / TX) * RY),
+                  J.Nz);
+  if (nbands > 1) {
+    hipLaunchKernelGGL((back_staged_band_kernel<real, FAN, RY, CA, TX>), grid, dim3(256), 0, st, J,
+                       part, nbands);
+    hipLaunchKernelGGL(back_reduce_kernel<real>, dim3((J.Nx + 15) / 16, (J.Ny + 15) / 16, J.Nz),
+                       dim3(256), 0, st, J, part, nbands);
+  } else {
+    hipLaunchKernelGGL((back_staged_kernel<real, FAN, RY, CA, TX>), grid, dim3(256), 0, st, J);
+  }
+}
+
+template <typename real, bool FAN, int RY, int CA>
+static void launch_staged(hipStream_t st, const Job<real> &J, int tx, int nbands, real *part) {
+  if (tx == 32)
+    launch_staged_tx<real, FAN, RY, CA, 32>(st, J, nbands, part);
+  else
+    launch_staged_tx<real, FAN, RY, CA, 16>(st, J, nbands, part);
+}
+
+
 template <typename real, bool FAN, int RY>
 static void launch_back(hipStream_t st, const Job<real> &J, int nbands, int stage, int tx,
                         real *part) {
   const dim3 grid(((J.Nx + 15) / 16) * nbands, (J.Ny + 16 * RY - 1) / (16 * RY), J.Nz);
   const bool buf = buffer_fits(sizeof(real), J.Ns, J.Na);
-  if (nbands == 1 && stage == 8) {
-    launch_staged<real, FAN, RY, 8>(st, J, tx);
-  } else if (nbands == 1 && stage == 16) {
-    launch_staged<real, FAN, RY, 16>(st, J, tx);
+  if (stage == 8) {
+    launch_staged<real, FAN, RY, 8>(st, J, tx, nbands, part);
+  } else if (stage == 16) {
+    launch_staged<real, FAN, RY, 16>(st, J, tx, nbands, part);
   } else if (nbands > 1) {
     if (buf)
       hipLaunchKernelGGL((back_band_kernel<real, FAN, RY, true>), grid, dim3(256), 0, st, J, part,
@@ -287,6 +313,7 @@ __global__ void single_line_fan_kernel(real *__restrict__ sino, Strides3 so, int
 // ---------------------------------------------------------------------------
 struct Layout {
   int Nx, Ny, Ns, Na, Nz;
+  bool fan;  // fanbeam geometry (the automatic launch shapes depend on it)
 };
 
 static int check_sizes(const Layout &L) {
@@ -301,15 +328,15 @@ static int check_sizes(const Layout &L) {
 static size_t elem_size(int dtype) { return dtype == GT_FLOAT64 ? 8 : 4; }
 
 // Launch shape of one call: the tuning knobs, or (GT_TUNE_AUTO, the default)
-// a choice by the work of the call, Nx*Ny*Na*Nz pixel-angles, from autotuner
-// trials on MI355X (256^2 to 32 x 2048^2, parallel and fan beam): small
-// problems need row or angle bands to fill the GPU; large ones run the
-// chunked forward and the LDS-staged backprojection.
+// a choice by the geometry and the work of the call, Nx*Ny*Na*Nz pixel-angles,
+// from autotuner trials on MI355X (256^2 to 32 x 2048^2, parallel and fan
+// beam): small problems need row or angle bands to fill the GPU; larger ones
+// run the chunked forward and the LDS-staged backprojection.
 struct FwdShape {
   int K, bands, chunk;  // bands and chunked reads need bin groups (K > 1)
 };
 struct BackShape {
-  int RY, bands, stage, tile;  // staging applies with one band
+  int RY, bands, stage, tile;  // stage > 0: LDS-staged, with or without angle bands
 };
 
 static double work_of(const Layout &L) { return (double)L.Nx * L.Ny * L.Na * L.Nz; }
@@ -321,16 +348,19 @@ static FwdShape fwd_shape(const Layout &L) {
     return {t.fwd_group, g ? t.fwd_bands : 1, g ? t.fwd_chunk : 1};
   }
   const double w = work_of(L);
-  if (w < 2e8) return {4, 8, 1};
+  if (L.fan && w < 2e8) return {4, 8, 1};
   if (w < 1.5e10) return {8, 8, 4};
   return {16, 8, 4};
 }
 
 static BackShape back_shape(const Layout &L) {
   const Tuning &t = tuning();
-  if (!t.automatic) return {t.back_rows, t.back_bands, t.back_bands > 1 ? 0 : t.back_stage, t.back_tile};
-  if (work_of(L) < 1e9) return {4, 8, 0, 16};
-  return {8, 1, 16, 32};
+  if (!t.automatic) return {t.back_rows, t.back_bands, t.back_stage, t.back_tile};
+  if (work_of(L) < 5e7) return {4, 8, 0, 16};
+  // Staged tiles hold 2048 pixels at either width; angle bands while the
+  // tiles of the call fill the GPU fewer than ~3 times (4096 tiles).
+  const int bands = (double)L.Nx * L.Ny * L.Nz < 4096.0 * 2048.0 ? 8 : 1;
+  return {8, bands, 16, L.fan ? 32 : 16};
 }
 
 // Chunked strip reads (chunk = 4) read 16 bytes per load and need both image
@@ -538,9 +568,8 @@ using namespace gt;
 
 extern "C" size_t gt_projection_workspace_bytes(int geometry, int adjoint, int dtype, int Nx,
                                                 int Ny, int Ns, int Na, int Nz, int in_order) {
-  (void)geometry;
   if (Nx <= 0 || Ny <= 0 || Ns <= 0 || Na <= 0 || Nz <= 0) return 0;
-  const Layout L{Nx, Ny, Ns, Na, Nz};
+  const Layout L{Nx, Ny, Ns, Na, Nz, geometry == GT_GEOMETRY_FAN};
   return adjoint ? back_ws(dtype, L) : fwd_ws(dtype, L, in_order);
 }
 
@@ -553,7 +582,7 @@ extern "C" int gt_radon_fwd(gt_stream_t stream, int dtype, void *sino, const voi
                             size_t workspace_bytes) {
   int rc = check_common(dtype, sino, img, ofs, geo, sino_order, img_order);
   if (rc) return rc;
-  const Layout L{Nx, Ny, Ns, Na, Nz};
+  const Layout L{Nx, Ny, Ns, Na, Nz, false};
   if ((rc = check_sizes(L))) return rc;
   hipStream_t st = static_cast<hipStream_t>(stream);
   return GT_DISPATCH(
@@ -573,7 +602,7 @@ extern "C" int gt_fan_fwd(gt_stream_t stream, int dtype, void *sino, const void 
   int rc = check_common(dtype, sino, img, ofs, geo, sino_order, img_order);
   if (rc) return rc;
   GT_REQUIRE(sdpd != nullptr, "NULL sdpd");
-  const Layout L{Nx, Ny, Ns, Na, Nz};
+  const Layout L{Nx, Ny, Ns, Na, Nz, true};
   if ((rc = check_sizes(L))) return rc;
   hipStream_t st = static_cast<hipStream_t>(stream);
   return GT_DISPATCH(
@@ -592,7 +621,7 @@ extern "C" int gt_radon_back(gt_stream_t stream, int dtype, void *img, const voi
                              size_t workspace_bytes) {
   int rc = check_common(dtype, img, sino, ofs, geo, img_order, sino_order);
   if (rc) return rc;
-  const Layout L{Nx, Ny, Ns, Na, Nz};
+  const Layout L{Nx, Ny, Ns, Na, Nz, false};
   if ((rc = check_sizes(L))) return rc;
   hipStream_t st = static_cast<hipStream_t>(stream);
   return GT_DISPATCH(
@@ -612,7 +641,7 @@ extern "C" int gt_fan_back(gt_stream_t stream, int dtype, void *img, const void 
   int rc = check_common(dtype, img, sino, ofs, geo, img_order, sino_order);
   if (rc) return rc;
   GT_REQUIRE(sdpd != nullptr, "NULL sdpd");
-  const Layout L{Nx, Ny, Ns, Na, Nz};
+  const Layout L{Nx, Ny, Ns, Na, Nz, true};
   if ((rc = check_sizes(L))) return rc;
   hipStream_t st = static_cast<hipStream_t>(stream);
   return GT_DISPATCH(

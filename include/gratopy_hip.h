@@ -55,9 +55,10 @@ int gt_version(void);
  *                      pattern (each XCD's L2 holds the padded sinogram rows of one band;
  *                      the workspace grows by B * Nx * Ny * Nz elements).
  *   GT_TUNE_BACK_STAGE: angles per chunk of the LDS-staged backprojection (0 = off, 8,
- *                      16; applies with one angle band): each block copies the sinogram
- *                      window its pixel tile reaches, per angle, into LDS and reads the
- *                      interpolation taps from there.
+ *                      16): each block copies the sinogram window its pixel tile
+ *                      reaches, per angle, into LDS and reads the interpolation taps from
+ *                      there; with GT_TUNE_BACK_BANDS > 1 each block sums one band of
+ *                      the angles.
  *   GT_TUNE_BACK_TILE: lanes along x of a staged backprojection tile (16 or 32; the
  *                      tile is that many pixels wide and 256 / width * rows-per-lane
  *                      pixels tall).

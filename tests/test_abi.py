@@ -109,6 +109,31 @@ def test_tuning_knobs_accept_their_values_and_reject_others(lib):
         autotune.set_variant(**before)
 
 
+def test_automatic_workspace_follows_the_automatic_shape(lib):
+    """In automatic mode the workspace query sizes what the chosen shape uses:
+    a small parallel-beam forward reads row-padded copies and a small fanbeam
+    one does not; the backprojection keeps angle-band partials only below
+    4096 * 2048 pixels."""
+    from gratopy_amd import autotune
+
+    before = autotune.current_variant()
+    try:
+        autotune.set_variant(automatic=1)
+        ws = lib.gt_projection_workspace_bytes
+        es, N, Ns, Na = 4, 100, 150, 30
+        radon_f, fan_f = ws(0, 0, 0, N, N, Ns, Na, 1, 0), ws(1, 0, 0, N, N, Ns, Na, 1, 0)
+        assert radon_f >= es * ((N + 4) * N * 2 + 8 * Ns * Na)
+        assert fan_f < es * (N + 4) * N * 2 + es * 8 * Ns * Na
+        small = ws(1, 1, 0, N, N, Ns, Na, 1, 0)
+        assert small >= es * 8 * N * N
+        big_n = 4096  # 1.7e7 pixels: staged, no bands
+        big = ws(1, 1, 0, big_n, big_n, Ns, 10, 1, 0)
+        assert big < es * 2 * big_n * big_n
+        assert ws(0, 1, 0, big_n, big_n, Ns, 10, 1, 0) == big
+    finally:
+        autotune.set_variant(**before)
+
+
 def test_batch_item_layout_matches_the_header(tmp_path):
     """ctypes' gt_batch_item has the C struct's size and field offsets (gcc on
     the header)."""

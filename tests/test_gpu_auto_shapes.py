@@ -1,9 +1,10 @@
-"""The automatic launch-shape choice (the library's default) across its size
-thresholds: stacks whose work Nx*Ny*Na*Nz puts each direction on every branch
-(bands only; banded 16-byte reads; K = 16; unstaged with angle bands; LDS
-staged) meet the parity bar slice by slice, and batch plans in automatic mode
-agree with the single-slice path to rounding (each picks its own shape, so
-they are not bitwise equal)."""
+"""The automatic launch-shape choice (the library's default) across its
+thresholds: stacks whose geometry, work Nx*Ny*Na*Nz and pixel count put each
+direction on every branch (forward: bands only, banded 16-byte reads, K = 16;
+backprojection: unstaged with angle bands, LDS-staged with and without angle
+bands, tiles 16 and 32 wide) meet the parity bar slice by slice, and batch
+plans in automatic mode agree with the single-slice path to rounding (each
+picks its own shape, so they are not bitwise equal)."""
 
 from __future__ import annotations
 
@@ -31,13 +32,14 @@ def automatic():
     autotune.set_variant(**before)
 
 
-# (geometry, N, angles, detectors, slices): work 2.4e7 (forward: bands only;
-# backprojection: angle bands), 1.2e9 (forward: banded 16-byte reads;
-# backprojection: staged), 1.6e10 (forward: K = 16)
+# (geometry, N, angles, detectors, slices) -> forward / backprojection shape
 STACKS = [
-    ("fan", 128, 90, 192, 16),
-    ("radon", 256, 360, 384, 50),
-    ("fan", 512, 720, 768, 85),
+    ("fan", 128, 90, 192, 16),     # work 2.4e7: bands only / unstaged, angle bands
+    ("radon", 128, 120, 192, 8),   # 1.6e7: banded 16-byte reads / unstaged, angle bands
+    ("fan", 256, 360, 384, 4),     # 9.4e7: bands only / staged, angle bands, tiles 32 wide
+    ("radon", 256, 360, 384, 50),  # 1.2e9: banded 16-byte reads / staged, angle bands, 16 wide
+    ("radon", 1024, 60, 1536, 9),  # 5.7e8, 9.4e6 pixels: staged without bands, 16 wide
+    ("fan", 512, 720, 768, 85),    # 1.6e10: K = 16 / staged without bands, 32 wide
 ]
 
 
