@@ -67,8 +67,9 @@ int gt_version(void);
  *                      the image (the workspace then holds two padded copies); 1 = one
  *                      pixel per load.
  *   GT_TUNE_AUTO:      1 (the default) = every call (and batch plan) picks its launch
- *                      shape from its size and the knobs above are ignored; 0 = the
- *                      knobs above apply.  Workspace sizes follow the same choice. */
+ *                      shape from its geometry and sizes and the knobs above are
+ *                      ignored; 0 = the knobs above apply.  Workspace sizes follow the
+ *                      same choice. */
 enum {
   GT_TUNE_FWD_GROUP = 0,
   GT_TUNE_BACK_ROWS = 1,
@@ -87,8 +88,11 @@ int gt_tune_get(int key);
 size_t gt_last_error(char *buf, size_t n);
 
 /* Workspace bytes needed by gt_{radon,fan}_{fwd,back} for these sizes.
+ * geometry is that of the call (GT_GEOMETRY_RADON or GT_GEOMETRY_FAN: the
+ * automatic launch shape, and with it the workspace, depends on it);
  * adjoint = 0 for the forward projection, 1 for the backprojection;
- * in_order is the order of the input array of that call. */
+ * in_order is the order of the input array of that call.  A call given less
+ * workspace than it needs returns GT_ENOSPC. */
 size_t gt_projection_workspace_bytes(int geometry, int adjoint, int dtype, int Nx, int Ny,
                                      int Ns, int Na, int Nz, int in_order);
 
