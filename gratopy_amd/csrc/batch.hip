@@ -97,13 +97,16 @@ __global__ __launch_bounds__(256) void batch_relayout_kernel(const Job<real> *__
   }
 }
 
+// K = 1: tile (slice, bin block, angle block), 64 bins x 4 angles.  K > 1:
+// tile (slice, block of the slice's groups as in the single-problem
+// launches, group_lane).
 template <typename real, int K, int CH = 1>
 __global__ __launch_bounds__(256) void batch_fwd_kernel(const Job<real> *__restrict__ jobs,
                                                         const int4 *__restrict__ tiles) {
   const int4 t = tiles[blockIdx.x];
   const Job<real> J = load_job(jobs, t.x);
-  const int a = t.z * 4 + threadIdx.y;
   if constexpr (K == 1) {
+    const int a = t.z * 4 + threadIdx.y;
     const int s = t.y * 64 + threadIdx.x;
     if (s >= J.Ns || a >= J.Na) return;
     if (J.fan)
@@ -112,33 +115,34 @@ __global__ __launch_bounds__(256) void batch_fwd_kernel(const Job<real> *__restr
       radon_fwd_ray(J, s, a, 0);
   } else {
     __shared__ real slots[Slots<K>::kCount * 256];
-    const int s0 = (t.y * 64 + threadIdx.x) * K;
-    if (s0 >= J.Ns || a >= J.Na) return;
+    int g;
+    const int a = group_lane<K>(t.y, J.Ns, J.Na, g);
+    if (a >= J.Na) return;
     real *acc = slots + threadIdx.y * 64 + threadIdx.x;
     if (J.fan)
-      fan_fwd_group<real, K, false, CH>(J, s0, a, 0, acc);
+      fan_fwd_group<real, K, false, CH>(J, g * K, a, 0, acc);
     else
-      radon_fwd_group<real, K, false, CH>(J, s0, a, 0, acc);
+      radon_fwd_group<real, K, false, CH>(J, g * K, a, 0, acc);
   }
 }
 
-// tile (slice, group block, angle block, band): one row band of the strip walks
-// of 64 x 4 bin groups; raw sums go to the slice's band partials
+// tile (slice, block of the slice's groups (group_lane), 0, band): one row
+// band of their strip walks; raw sums go to the slice's band partials
 template <typename real, int K, int CH, int NB>
 __global__ __launch_bounds__(256) void batch_fwd_band_kernel(const Job<real> *__restrict__ jobs,
                                                              const int4 *__restrict__ tiles) {
   __shared__ real slots[Slots<K>::kCount * 256];
   const int4 t = tiles[blockIdx.x];
   const Job<real> J = load_job(jobs, t.x);
-  const int a = t.z * 4 + threadIdx.y;
-  const int s0 = (t.y * 64 + threadIdx.x) * K;
-  if (s0 >= J.Ns || a >= J.Na) return;
+  int g;
+  const int a = group_lane<K>(t.y, J.Ns, J.Na, g);
+  if (a >= J.Na) return;
   real *acc = slots + threadIdx.y * 64 + threadIdx.x;
   const Band<real> B{t.w, NB, J.part};
   if (J.fan)
-    fan_fwd_group<real, K, true, CH>(J, s0, a, 0, acc, B);
+    fan_fwd_group<real, K, true, CH>(J, g * K, a, 0, acc, B);
   else
-    radon_fwd_group<real, K, true, CH>(J, s0, a, 0, acc, B);
+    radon_fwd_group<real, K, true, CH>(J, g * K, a, 0, acc, B);
 }
 
 // tile (slice, angle): the bands of one sinogram row added in band order, then
@@ -218,7 +222,9 @@ static int check_item(const gt_batch_item &it) {
   GT_REQUIRE(it.geometry == GT_GEOMETRY_RADON || it.geometry == GT_GEOMETRY_FAN,
              "batch item: unknown geometry");
   GT_REQUIRE(it.Nx > 0 && it.Ny > 0 && it.Ns > 0 && it.Na > 0, "batch item: sizes must be positive");
-  GT_REQUIRE(it.Na <= 65535 && (int64_t)it.Nx * it.Ny < ((int64_t)1 << 31),
+  // sizes index with 32-bit lanes: pixels, and (angle, bin) pairs of a slice
+  GT_REQUIRE(it.Na <= 65535 && (int64_t)it.Nx * it.Ny < ((int64_t)1 << 31) &&
+                 (int64_t)it.Na * it.Ns < ((int64_t)1 << 31) - 256,
              "batch item: too large");
   GT_REQUIRE(it.ofs && it.geo, "batch item: NULL table");
   GT_REQUIRE(it.geometry == GT_GEOMETRY_RADON || it.sdpd, "batch item: NULL sdpd");
@@ -277,14 +283,19 @@ static int plan_create(gt_batch_plan **out, int dtype, int n, const gt_batch_ite
   TileSet sets[5];
   for (int i = 0; i < n; ++i) {
     const gt_batch_item &it = items[i];
-    const int nsb = (it.Ns + 64 * K - 1) / (64 * K), nab = (it.Na + 3) / 4;
     // forward tile: ~ rows crossed x strip width per ray group
     const double cf =
         (double)std::max(it.Nx, it.Ny) * (K + 1) * (it.geometry == GT_GEOMETRY_FAN ? 1.5 : 1.0);
     // bands innermost: the 8 bands of a tile are dispatched to the 8 XCDs in turn
-    for (int b = 0; b < nsb; ++b)
-      for (int a = 0; a < nab; ++a)
-        for (int w = 0; w < nb; ++w) sets[kForward].add(i, b, a, cf / nb, w);
+    if (K > 1) {  // blocks of 256 lanes over (angle, group) pairs (group_lane)
+      const int64_t nblk = group_blocks(it.Ns, it.Na, K);
+      for (int64_t b = 0; b < nblk; ++b)
+        for (int w = 0; w < nb; ++w) sets[kForward].add(i, (int)b, 0, cf / nb, w);
+    } else {  // 64 bins x 4 angles
+      const int nsb = (it.Ns + 63) / 64, nab = (it.Na + 3) / 4;
+      for (int b = 0; b < nsb; ++b)
+        for (int a = 0; a < nab; ++a) sets[kForward].add(i, b, a, cf);
+    }
     for (int bx = 0; bx < (it.Nx + 31) / 32; ++bx)
       for (int by = 0; by < (it.Ny + 31) / 32; ++by) sets[kRelayout].add(i, bx * 32, by * 32, 1.0);
     for (int a = 0; a < it.Na; ++a) sets[kStage].add(i, a, 0, (double)it.Ns);
@@ -573,7 +584,7 @@ extern "C" int gt_batch_plan_create(gt_batch_plan **plan, int dtype, int n_items
   // (128^2..1024^2, 60..720 angles); batches have no backprojection angle bands
   Tuning t = tuning();
   if (t.automatic) {
-    t.fwd_group = 8;
+    t.fwd_group = 16;
     t.fwd_bands = 8;
     t.fwd_chunk = 4;
     t.back_rows = 8;

@@ -449,6 +449,38 @@ __device__ __forceinline__ void band_rows(int Nyy, int index, int count, int &yl
   yhi = yhi < r1 ? yhi : r1;
 }
 
+// Lane mapping of the bin-group forward launches (256-lane blocks of 64 x 4;
+// groups = ceil(Ns / K) bin groups per angle).  The whole columns of 64
+// groups come first, one block per (column, 4 angles): its 4 waves walk
+// nearly the same pixels, which the CU's cache then serves once.  The
+// groups left over (groups % 64 of them: 32 of 160 with K = 16 and 2560
+// bins; all of a small slice's) are packed over consecutive angles, lane
+// (a * tail + j) taking group cols*64 + j of angle a, so their waves are
+// full instead of mostly idle and a block holds the same detector edge for
+// a run of adjacent angles.  Returns a (>= Na for lanes past the end).
+template <int K>
+__device__ __forceinline__ int group_lane(int block, int Ns, int Na, int &g) {
+  const int groups = (Ns + K - 1) / K;
+  const int cols = groups / 64;
+  const int nfb = cols * ((Na + 3) / 4);
+  if (block < nfb) {
+    const int ab = block / cols;
+    g = (block - ab * cols) * 64 + threadIdx.x;
+    return ab * 4 + threadIdx.y;
+  }
+  const int tail = groups - cols * 64;
+  const int lin = (block - nfb) * 256 + threadIdx.y * 64 + threadIdx.x;
+  const int a = lin / tail;
+  g = cols * 64 + (lin - a * tail);
+  return a;
+}
+
+// Blocks of a bin-group forward launch over one slice (group_lane).
+inline int64_t group_blocks(int Ns, int Na, int K) {
+  const int64_t groups = (Ns + K - 1) / K, cols = groups / 64, tail = groups - cols * 64;
+  return cols * ((Na + 3) / 4) + (Na * tail + 255) / 256;
+}
+
 // acc: this lane's slot 0 in an LDS array of Slots<K>::kCount rows of 256.
 // CH > 1: the strip row is read CH pixels per load.  The last load of a row
 // may reach up to CH - 1 pixels past the strip: inside the row those lie

@@ -3,8 +3,9 @@
 // row ranges, bin groups, backprojection gather) live in kernels.h and are
 // shared with the batched launches.
 //
-// Forward: a block is 64 lanes (detector bins or bin groups) x 4 angles,
-// grid z = slice.  Rays closer to the x axis read an x-fastest copy of the
+// Forward: one lane per detector bin (64 bins x 4 angles per block) or per
+// bin group (256 consecutive (angle, group) pairs per block); the slice is
+// the last grid axis.  Rays closer to the x axis read an x-fastest copy of the
 // image, the others a y-fastest copy, so a wave always reads along the
 // fastest axis (gratopy/radon.cl:66-161, gratopy/fanbeam.cl:72-256).
 // Backward: a block is 16 x 16*RY pixels; a prologue builds the per-angle
@@ -31,48 +32,49 @@ static void launch_relayout(hipStream_t st, real *dst, Strides3 ds, const real *
   hipLaunchKernelGGL(relayout_kernel<real>, grid, dim3(256), 0, st, dst, ds, src, ss, N0, N1);
 }
 
-// K = 1: one lane per detector bin (the reference's loops); K > 1: a lane
-// owns bins [K*i, K*i + K) and reads its strip rows CH pixels per load.
-template <typename real, bool FAN, int K, int CH = 1>
-__global__ __launch_bounds__(256) void fwd_kernel(Job<real> J) {
+// One lane per detector bin (the reference's loops): 64 bins x 4 angles.
+template <typename real, bool FAN>
+__global__ __launch_bounds__(256) void fwd_ray_kernel(Job<real> J) {
   const int a = blockIdx.y * 4 + threadIdx.y;
-  const int z = blockIdx.z;
-  if constexpr (K == 1) {
-    const int s = blockIdx.x * 64 + threadIdx.x;
-    if (s >= J.Ns || a >= J.Na) return;
-    if (FAN)
-      fan_fwd_ray(J, s, a, z);
-    else
-      radon_fwd_ray(J, s, a, z);
-  } else {
-    __shared__ real slots[Slots<K>::kCount * 256];
-    const int s0 = (blockIdx.x * 64 + threadIdx.x) * K;
-    if (s0 >= J.Ns || a >= J.Na) return;
-    real *acc = slots + threadIdx.y * 64 + threadIdx.x;
-    if (FAN)
-      fan_fwd_group<real, K, false, CH>(J, s0, a, z, acc);
-    else
-      radon_fwd_group<real, K, false, CH>(J, s0, a, z, acc);
-  }
+  const int s = blockIdx.x * 64 + threadIdx.x;
+  if (s >= J.Ns || a >= J.Na) return;
+  if (FAN)
+    fan_fwd_ray(J, s, a, blockIdx.z);
+  else
+    radon_fwd_ray(J, s, a, blockIdx.z);
 }
 
-// Banded form of the bin-group forward: blockIdx.x = band + nbands * group,
+// Bin groups: a lane owns bins [K*g, K*g + K) of angle a (group_lane) and
+// reads its strip rows CH pixels per load.
+template <typename real, bool FAN, int K, int CH>
+__global__ __launch_bounds__(256) void fwd_group_kernel(Job<real> J) {
+  __shared__ real slots[Slots<K>::kCount * 256];
+  int g;
+  const int a = group_lane<K>(blockIdx.x, J.Ns, J.Na, g);
+  if (a >= J.Na) return;
+  real *acc = slots + threadIdx.y * 64 + threadIdx.x;
+  if (FAN)
+    fan_fwd_group<real, K, false, CH>(J, g * K, a, blockIdx.y, acc);
+  else
+    radon_fwd_group<real, K, false, CH>(J, g * K, a, blockIdx.y, acc);
+}
+
+// Banded form of the bin-group forward: blockIdx.x = band + nbands * block,
 // so consecutive blocks, which the dispatcher hands to the 8 XCDs in turn,
 // walk different row bands and each XCD's L2 serves one band of the image.
-template <typename real, bool FAN, int K, int CH = 1>
+template <typename real, bool FAN, int K, int CH>
 __global__ __launch_bounds__(256) void fwd_band_kernel(Job<real> J, real *part, int nbands) {
   __shared__ real slots[Slots<K>::kCount * 256];
   const int band = blockIdx.x % nbands;
-  const int s0 = ((blockIdx.x / nbands) * 64 + threadIdx.x) * K;
-  const int a = blockIdx.y * 4 + threadIdx.y;
-  const int z = blockIdx.z;
-  if (s0 >= J.Ns || a >= J.Na) return;
+  int g;
+  const int a = group_lane<K>(blockIdx.x / nbands, J.Ns, J.Na, g);
+  if (a >= J.Na) return;
   real *acc = slots + threadIdx.y * 64 + threadIdx.x;
   const Band<real> B{band, nbands, part};
   if (FAN)
-    fan_fwd_group<real, K, true, CH>(J, s0, a, z, acc, B);
+    fan_fwd_group<real, K, true, CH>(J, g * K, a, blockIdx.y, acc, B);
   else
-    radon_fwd_group<real, K, true, CH>(J, s0, a, z, acc, B);
+    radon_fwd_group<real, K, true, CH>(J, g * K, a, blockIdx.y, acc, B);
 }
 
 template <typename real>
@@ -349,7 +351,7 @@ static FwdShape fwd_shape(const Layout &L) {
   }
   const double w = work_of(L);
   if (L.fan && w < 2e8) return {4, 8, 1};
-  if (w < 1.5e10) return {8, 8, 4};
+  if (w < 2e9) return {8, 8, 4};
   return {16, 8, 4};
 }
 
@@ -419,20 +421,23 @@ __global__ __launch_bounds__(256) void pad_rows_kernel(real *__restrict__ dst, i
 }
 
 template <typename real, bool FAN, int K>
-static void launch_fwd_group(hipStream_t st, const Job<real> &J, dim3 grid, int nb, bool chunk,
-                             real *part) {
+static int launch_fwd_group(hipStream_t st, const Job<real> &J, int nb, bool chunk, real *part) {
   constexpr int CH = 16 / sizeof(real);
-  const dim3 block(64, 4);
+  const int64_t blocks = group_blocks(J.Ns, J.Na, K);
+  GT_REQUIRE(blocks * 256 < ((int64_t)1 << 31) && blocks * nb < ((int64_t)1 << 31),
+             "too many detector groups");
+  const dim3 grid((unsigned)(blocks * nb), J.Nz), block(64, 4);
   if (nb > 1) {
     if (chunk)
       hipLaunchKernelGGL((fwd_band_kernel<real, FAN, K, CH>), grid, block, 0, st, J, part, nb);
     else
       hipLaunchKernelGGL((fwd_band_kernel<real, FAN, K, 1>), grid, block, 0, st, J, part, nb);
   } else if (chunk) {
-    hipLaunchKernelGGL((fwd_kernel<real, FAN, K, CH>), grid, block, 0, st, J);
+    hipLaunchKernelGGL((fwd_group_kernel<real, FAN, K, CH>), grid, block, 0, st, J);
   } else {
-    hipLaunchKernelGGL((fwd_kernel<real, FAN, K, 1>), grid, block, 0, st, J);
+    hipLaunchKernelGGL((fwd_group_kernel<real, FAN, K, 1>), grid, block, 0, st, J);
   }
+  return GT_OK;
 }
 
 template <typename real, bool FAN>
@@ -489,25 +494,27 @@ static int fwd_impl(hipStream_t st, real *sino, const real *img, const real *ofs
   J.so = order_strides(sino_order, L.Ns, L.Na, L.Nz);
   const int K = sh.K, nb = sh.bands;
   const bool chunk = sh.chunk > 1;
-  const int groups = (L.Ns + 64 * K - 1) / (64 * K);
   real *part = nullptr;
   if (nb > 1) {
-    GT_REQUIRE((int64_t)groups * nb < ((int64_t)1 << 31), "too many detector groups");
     // the band partials are the last part of the workspace (after the image copies)
     const size_t part_bytes = align_up(sizeof(real) * (size_t)nb * L.Ns * L.Na * L.Nz);
     part = reinterpret_cast<real *>(static_cast<char *>(ws) + (need - part_bytes));
   }
-  const dim3 grid(groups * nb, (L.Na + 3) / 4, L.Nz);
-  if (K == 16)
-    launch_fwd_group<real, FAN, 16>(st, J, grid, nb, chunk, part);
-  else if (K == 8)
-    launch_fwd_group<real, FAN, 8>(st, J, grid, nb, chunk, part);
-  else if (K == 4)
-    launch_fwd_group<real, FAN, 4>(st, J, grid, nb, chunk, part);
-  else if (K == 2)
-    launch_fwd_group<real, FAN, 2>(st, J, grid, nb, chunk, part);
-  else
-    hipLaunchKernelGGL((fwd_kernel<real, FAN, 1>), grid, dim3(64, 4), 0, st, J);
+  if (K == 1) {
+    const dim3 grid((L.Ns + 63) / 64, (L.Na + 3) / 4, L.Nz);
+    hipLaunchKernelGGL((fwd_ray_kernel<real, FAN>), grid, dim3(64, 4), 0, st, J);
+  } else {
+    int rc;
+    if (K == 16)
+      rc = launch_fwd_group<real, FAN, 16>(st, J, nb, chunk, part);
+    else if (K == 8)
+      rc = launch_fwd_group<real, FAN, 8>(st, J, nb, chunk, part);
+    else if (K == 4)
+      rc = launch_fwd_group<real, FAN, 4>(st, J, nb, chunk, part);
+    else
+      rc = launch_fwd_group<real, FAN, 2>(st, J, nb, chunk, part);
+    if (rc) return rc;
+  }
   if (nb > 1)
     hipLaunchKernelGGL(band_reduce_kernel<real>, dim3((L.Ns + 255) / 256, L.Na, L.Nz), dim3(256),
                        0, st, J, part, nb);
