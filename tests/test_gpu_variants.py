@@ -87,6 +87,33 @@ def test_forward_variants(variant, restore_variant):
         _check_wide(ps, np.random.default_rng(4).random(shape + (2,)), False, (geometry, nd, variant))
 
 
+GROUP_VARIANTS = [v for v in autotune.FWD_VARIANTS if v["fwd_group"] > 1]
+
+
+@pytest.mark.parametrize("variant", GROUP_VARIANTS,
+                         ids=[autotune.variant_key(v) for v in GROUP_VARIANTS])
+def test_forward_lane_mapping(variant, restore_variant):
+    """Bin-group launches cover every (angle, group) pair once: whole columns
+    of 64 groups per 4 angles, then the leftover groups of all angles packed
+    into full waves.  Detector counts give whole columns with a leftover
+    (1124), whole columns only (1024 for K = 16; 1024 / K groups otherwise)
+    and leftovers only or mostly (200); angle counts below, at and past a
+    block of 4, so the last block of either kind is partly past the end.
+    With this many bins per pixel float32 sits at its own floor, so it is
+    held to the reference algorithm's float32 distance (as the wide cases)."""
+    print(f"variant {autotune.variant_key(variant)} lane mapping", flush=True)
+    autotune.set_variant(**variant)
+    shape = (24, 20)
+    img = np.random.default_rng(6).random(shape + (2,))
+    for geometry in ("radon", "fan"):
+        for nd in (200, 1024, 1124):
+            kw = dict(R=60.0, RE=25.0, detector_width=70.0) if geometry == "fan" else {}
+            for na in (1, 3, 6):
+                g = gratopy.RADON if geometry == "radon" else gratopy.FANBEAM
+                ps = gratopy.ProjectionSettings("cuda:0", g, shape, na, nd, **kw)
+                _check_wide(ps, img, False, (geometry, nd, na))
+
+
 @pytest.mark.parametrize("variant", autotune.BACK_VARIANTS, ids=BACK_IDS)
 def test_backward_variants(variant, restore_variant):
     print(f"variant {autotune.variant_key(variant)}", flush=True)
