@@ -526,7 +526,9 @@ __device__ __forceinline__ void fan_fwd_group(const Job<real> &J, int s0, int a,
   const double ud0 = (double)ylo - (double)F.midy;
   const double t00 = tdy * ud0 + (double)F.RE - tdx * (double)F.midx;
   const double q00 = qdy * ud0 + c0 * (double)F.RE - qdx * (double)F.midx;
-  for (int y = ylo; y <= yhi; ++y) {
+  // row y starts at F.base + y * F.row_stride, walked by one add per row
+  const real *row = F.base + (int64_t)ylo * F.row_stride;
+  for (int y = ylo; y <= yhi; ++y, row += F.row_stride) {
     const real u = (real)y - F.midy;
     const real xhigh = F.qx + F.dpx * (u - F.qy);
     const real xlow = F.qx + F.dmx * (u - F.qy);
@@ -537,7 +539,6 @@ __device__ __forceinline__ void fan_fwd_group(const Job<real> &J, int s0, int a,
     real t = (real)fma(tdx, xd, fma(tdy, dyr, t00));
     real q = (real)fma(qdx, xd, fma(qdy, dyr, q00));
     // walked by address: the end pointer doubles as the loop bound
-    const real *row = F.base + (int64_t)y * F.row_stride;
     const real *end = row + xh;
     for (const real *px = row + xl; px <= end; px += CH) {
       const RowChunk<real, CH> c = load_chunk<real, CH>(px);
@@ -607,7 +608,9 @@ __device__ __forceinline__ void radon_fwd_group(const Job<real> &J, int s0, int 
   const real lo = (real)0.5, hi = (real)K + (real)2.5;
   const real sl = (real)(s0 - 1), sh = (real)(s0 + K);
   const double p00 = (double)off - (double)s0 + 2.0;  // pixel (0, 0) relative to bin s0 - 2
-  for (int y = ylo; y <= yhi; ++y) {
+  // row y starts at base + y * row_stride, walked by one add per row
+  const real *row = base + (int64_t)ylo * row_stride;
+  for (int y = ylo; y <= yhi; ++y, row += row_stride) {
     const real d = (real)y * Y + off;  // detector position of (0, y)
     int xl = (int)((sl - d) * inv);
     int xh = (int)((sh - d) * inv);
@@ -623,7 +626,6 @@ __device__ __forceinline__ void radon_fwd_group(const Job<real> &J, int s0, int 
     // evaluated in double: x*X and y*Y + off are up to Ns in size and nearly
     // cancel; inside the strip it lies in [1, K+2], so truncation is floor
     real pr = (real)fma((double)X, (double)xl, fma((double)Y, (double)y, p00));
-    const real *row = base + (int64_t)y * row_stride;
     const real *end = row + xh;
     for (const real *px = row + xl; px <= end; px += CH) {
       const RowChunk<real, CH> c = load_chunk<real, CH>(px);
