@@ -16,20 +16,34 @@
 
 namespace gt {
 
+// pad > 0: the copy's rows along its fast axis are pad elements longer than
+// the image; the blocks at the end of the fast axis also zero those elements
+// of their 32 rows (so no separate fill of the padding is needed).
 template <typename real>
 __global__ __launch_bounds__(256) void relayout_kernel(real *__restrict__ dst, Strides3 ds,
                                                        const real *__restrict__ src, Strides3 ss,
-                                                       int N0, int N1) {
+                                                       int N0, int N1, int pad) {
   __shared__ real tile[32][33];
-  relayout_tile(tile, dst, ds, src, ss, N0, N1, (int64_t)blockIdx.z, blockIdx.x * 32,
-                blockIdx.y * 32);
+  const int64_t z = blockIdx.z;
+  relayout_tile(tile, dst, ds, src, ss, N0, N1, z, blockIdx.x * 32, blockIdx.y * 32);
+  if (pad) {
+    const bool fast0 = ds.s0 <= ds.s1;
+    const int nf = fast0 ? N0 : N1, ns = fast0 ? N1 : N0;
+    const int bf = (fast0 ? blockIdx.x : blockIdx.y) * 32, bs = (fast0 ? blockIdx.y : blockIdx.x) * 32;
+    if (bf + 32 >= nf) {
+      for (int t = threadIdx.x; t < 32 * pad; t += 256) {
+        const int r = bs + t / pad, f = nf + t % pad;
+        if (r < ns) dst[(fast0 ? f : r) * ds.s0 + (fast0 ? r : f) * ds.s1 + z * ds.s2] = 0;
+      }
+    }
+  }
 }
 
 template <typename real>
 static void launch_relayout(hipStream_t st, real *dst, Strides3 ds, const real *src, Strides3 ss,
-                            int N0, int N1, int N2) {
+                            int N0, int N1, int N2, int pad = 0) {
   dim3 grid((N0 + 31) / 32, (N1 + 31) / 32, N2);
-  hipLaunchKernelGGL(relayout_kernel<real>, grid, dim3(256), 0, st, dst, ds, src, ss, N0, N1);
+  hipLaunchKernelGGL(relayout_kernel<real>, grid, dim3(256), 0, st, dst, ds, src, ss, N0, N1, pad);
 }
 
 // One lane per detector bin (the reference's loops): 64 bins x 4 angles.
@@ -84,15 +98,12 @@ __global__ __launch_bounds__(256) void band_reduce_kernel(Job<real> J, const rea
   if (s < J.Ns) band_reduce_entry(J, part, nbands, s, blockIdx.y, blockIdx.z);
 }
 
-template <typename real>
-__global__ void back_table_kernel(AngleRec<real> *__restrict__ tab, const real *__restrict__ ofs,
-                                  const real *__restrict__ geo, int fan, int Na) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a < Na) back_table_entry(tab, ofs, geo, fan, a);
-}
-
+// Backprojection prologue, block (j block, angle, slice): the zero-padded,
+// pre-weighted sinogram row, and (first block of the angle's row in slice 0)
+// the angle's affine table entry.
 template <typename real>
 __global__ __launch_bounds__(256) void stage_sino_kernel(real *__restrict__ S,
+                                                         AngleRec<real> *__restrict__ tab,
                                                          const real *__restrict__ sino,
                                                          Strides3 si, const real *__restrict__ ofs,
                                                          const real *__restrict__ sdpd,
@@ -100,6 +111,7 @@ __global__ __launch_bounds__(256) void stage_sino_kernel(real *__restrict__ S,
                                                          int Ns, int Na) {
   const int j = blockIdx.x * 256 + threadIdx.x;
   if (j < Ns + 4) stage_entry(S, sino, si, ofs, sdpd, geo, fan, Ns, Na, j, blockIdx.y, blockIdx.z);
+  if (j == 0 && blockIdx.z == 0) back_table_entry(tab, ofs, geo, fan, blockIdx.y);
 }
 
 template <typename real, bool FAN, int RY, bool BUF>
@@ -412,14 +424,6 @@ static Job<real> make_job(bool fan, const Layout &L, const real *ofs, const real
   return J;
 }
 
-// Zeros columns [n0, n0 + kRowPad) of every row of a row-padded copy.
-template <typename real>
-__global__ __launch_bounds__(256) void pad_rows_kernel(real *__restrict__ dst, int64_t row,
-                                                       int64_t rows, int n0) {
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i < rows * kRowPad) dst[(i / kRowPad) * row + n0 + i % kRowPad] = 0;
-}
-
 template <typename real, bool FAN, int K>
 static int launch_fwd_group(hipStream_t st, const Job<real> &J, int nb, bool chunk, real *part) {
   constexpr int CH = 16 / sizeof(real);
@@ -461,13 +465,8 @@ static int fwd_impl(hipStream_t st, real *sino, const real *img, const real *ofs
     real *cx = reinterpret_cast<real *>(w);
     w += align_up(sizeof(real) * (size_t)rx * L.Ny * L.Nz);
     real *cy = reinterpret_cast<real *>(w);
-    launch_relayout<real>(st, cx, Strides3{1, rx, rx * L.Ny}, img, is, L.Nx, L.Ny, L.Nz);
-    launch_relayout<real>(st, cy, Strides3{ry, 1, ry * L.Nx}, img, is, L.Nx, L.Ny, L.Nz);
-    const int64_t nrx = (int64_t)L.Ny * L.Nz, nry = (int64_t)L.Nx * L.Nz;
-    hipLaunchKernelGGL(pad_rows_kernel<real>, dim3((unsigned)((nrx * kRowPad + 255) / 256)),
-                       dim3(256), 0, st, cx, rx, nrx, L.Nx);
-    hipLaunchKernelGGL(pad_rows_kernel<real>, dim3((unsigned)((nry * kRowPad + 255) / 256)),
-                       dim3(256), 0, st, cy, ry, nry, L.Ny);
+    launch_relayout<real>(st, cx, Strides3{1, rx, rx * L.Ny}, img, is, L.Nx, L.Ny, L.Nz, kRowPad);
+    launch_relayout<real>(st, cy, Strides3{ry, 1, ry * L.Nx}, img, is, L.Nx, L.Ny, L.Nz, kRowPad);
     J.imgX = cx;
     J.imgY = cy;
     J.rowX = rx;
@@ -531,11 +530,9 @@ static int back_impl(hipStream_t st, real *img, const real *sino, const real *of
   char *w = static_cast<char *>(ws);
   AngleRec<real> *tab = reinterpret_cast<AngleRec<real> *>(w);
   real *S = reinterpret_cast<real *>(w + align_up(sizeof(real) * 8 * (size_t)L.Na));
-  hipLaunchKernelGGL(back_table_kernel<real>, dim3((L.Na + 63) / 64), dim3(64), 0, st, tab, ofs,
-                     geo, (int)FAN, L.Na);
   const Strides3 si = order_strides(sino_order, L.Ns, L.Na, L.Nz);
   hipLaunchKernelGGL(stage_sino_kernel<real>, dim3((L.Ns + 4 + 255) / 256, L.Na, L.Nz), dim3(256),
-                     0, st, S, sino, si, ofs, sdpd, geo, (int)FAN, L.Ns, L.Na);
+                     0, st, S, tab, sino, si, ofs, sdpd, geo, (int)FAN, L.Ns, L.Na);
   Job<real> J = make_job<real>(FAN, L, ofs, sdpd, geo);
   J.img = img;
   J.io = order_strides(img_order, L.Nx, L.Ny, L.Nz);
