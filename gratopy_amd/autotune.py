@@ -3,14 +3,15 @@
 By default every call picks its launch shape from its size (``automatic``).
 Knobs (the C ABI's ``GT_TUNE_*`` keys): the forward kernels assign K
 adjacent detector bins to a lane (``fwd_group`` K in 1, 2, 4, 8, 16; K = 1 is the
-per-ray kernel), optionally split the strip walk into 8 row bands
-(``fwd_bands``; one band per XCD, partial sums added in band order) and read
-strip rows 16 bytes per load (``fwd_chunk`` = 4).  The backprojection assigns
+per-ray kernel), optionally split the strip walk into 8 or 32 row bands
+(``fwd_bands``; with 8, one band per XCD; 32 give small images four times the
+blocks; partial sums added in band order) and read strip rows 16 bytes per
+load (``fwd_chunk`` = 4).  The backprojection assigns
 RY pixels along y to a lane (``back_rows`` in 1, 2, 4, 8), sums all angles per
 tile or splits them into 8 angle bands (``back_bands``), and reads its taps
 from memory or stages each tile's sinogram windows in LDS for chunks of 8 or
 16 angles (``back_stage``) in tiles 16 or 32 pixels wide (``back_tile``).
-Batched launches take all but the angle bands.  :func:`tune` times every variant on the
+Batched launches take all but the angle bands and take at most 8 row bands.  :func:`tune` times every variant on the
 caller's own geometry and array shapes with HIP events, keeps the fastest,
 and remembers it per shape key in a JSON cache
 (``$GRATOPY_AMD_TUNE_CACHE``, default ``~/.cache/gratopy_amd/tune.json``).
@@ -40,7 +41,7 @@ DEFAULTS = {"fwd_group": 4, "back_rows": 1, "fwd_bands": 1, "back_bands": 1, "ba
 
 FWD_GROUPS = (1, 2, 4, 8, 16)
 BACK_ROWS = (1, 2, 4, 8)
-FWD_BANDS = (1, 8)
+FWD_BANDS = (1, 8, 32)
 FWD_CHUNKS = (1, 4)
 BACK_STAGES = (0, 8, 16)
 BACK_TILES = (16, 32)
@@ -197,6 +198,8 @@ def tune_batch(settings, imgs, sinos, repeats: int = 2) -> dict:
 
     fwd_times, back_times = {}, {}
     for v in FWD_VARIANTS:
+        if v["fwd_bands"] > 8:  # batch plans take at most 8 row bands
+            continue
         set_variant(**v)
         fwd_times[variant_key(v)] = (timed(_hip.BATCH_FORWARD), v)
     for v in BACK_VARIANTS:

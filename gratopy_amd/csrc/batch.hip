@@ -583,6 +583,7 @@ extern "C" int gt_batch_plan_create(gt_batch_plan **plan, int dtype, int n_items
   // automatic launch shapes: the batch autotuner's choice on 64 uneven slices
   // (128^2..1024^2, 60..720 angles); batches have no backprojection angle bands
   Tuning t = tuning();
+  t.fwd_bands = std::min(t.fwd_bands, 8);  // batch forwards take up to 8 row bands
   if (t.automatic) {
     t.fwd_group = 16;
     t.fwd_bands = 8;

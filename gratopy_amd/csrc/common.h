@@ -25,7 +25,7 @@ inline size_t align_up(size_t n, size_t a = 256) { return (n + a - 1) / a * a; }
 struct Tuning {
   int fwd_group = 4;  // detector bins per lane in the forward kernels (1, 2, 4, 8, 16)
   int back_rows = 1;  // pixels per lane (along y) in the backprojection (1, 2, 4, 8)
-  int fwd_bands = 1;  // row bands of the forward strip walk (1, 2, 4, 8; used when fwd_group > 1)
+  int fwd_bands = 1;  // row bands of the forward strip walk (1, 2, 4, 8, 16, 32; fwd_group > 1)
   int back_bands = 1;  // angle bands of the backprojection (1, 2, 4, 8)
   int back_stage = 0;  // angles per LDS staging chunk of the backprojection (0 = unstaged, 8, 16)
   int back_tile = 16;  // lanes along x of a staged backprojection tile (16, 32)

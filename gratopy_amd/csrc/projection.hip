@@ -12,6 +12,8 @@
 // affine table and the zero-padded, pre-weighted sinogram
 // (gratopy/radon.cl:183-241, gratopy/fanbeam.cl:290-391).
 
+#include <algorithm>
+
 #include "kernels.h"
 
 namespace gt {
@@ -362,9 +364,12 @@ static FwdShape fwd_shape(const Layout &L) {
     return {t.fwd_group, g ? t.fwd_bands : 1, g ? t.fwd_chunk : 1};
   }
   const double w = work_of(L);
-  if (L.fan && w < 2e8) return {4, 8, 1};
-  if (w < 2e9) return {8, 8, 4};
-  return {16, 8, 4};
+  if (L.fan && w < 5e7) return {4, 8, 1};
+  const int K = w < 2e9 ? 8 : 16;
+  // 32 row bands where 8 would launch fewer than 4096 blocks (a single slice
+  // leaves the GPU part idle), with at least 16 rows per band
+  const bool more = group_blocks(L.Ns, L.Na, K) * 8 * L.Nz < 4096 && std::min(L.Nx, L.Ny) >= 512;
+  return {K, more ? 32 : 8, 4};
 }
 
 static BackShape back_shape(const Layout &L) {

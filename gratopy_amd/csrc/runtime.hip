@@ -45,8 +45,8 @@ extern "C" int gt_tune_set(int key, int value) {
       t.back_rows = value;
       return GT_OK;
     case GT_TUNE_FWD_BANDS:
-      if (value != 1 && value != 2 && value != 4 && value != 8)
-        return gt::fail(GT_EINVAL, "forward row bands must be 1, 2, 4 or 8");
+      if (value != 1 && value != 2 && value != 4 && value != 8 && value != 16 && value != 32)
+        return gt::fail(GT_EINVAL, "forward row bands must be 1, 2, 4, 8, 16 or 32");
       t.fwd_bands = value;
       return GT_OK;
     case GT_TUNE_BACK_BANDS:

@@ -46,11 +46,12 @@ int gt_version(void);
  * rounding.
  *   GT_TUNE_FWD_GROUP: detector bins per lane in the forward projection (1, 2, 4, 8, 16)
  *   GT_TUNE_BACK_ROWS: pixels per lane in the backprojection (1, 2, 4, 8)
- *   GT_TUNE_FWD_BANDS: row bands of the forward strip walk (1, 2, 4, 8; applies when
- *                      GT_TUNE_FWD_GROUP > 1).  With B > 1 bands, consecutive thread
- *                      blocks take different bands (so each XCD's L2 holds one band of
- *                      the image), and the workspace grows by B * Ns * Na * Nz elements
- *                      for the per-band partial sums.
+ *   GT_TUNE_FWD_BANDS: row bands of the forward strip walk (1, 2, 4, 8, 16, 32; applies
+ *                      when GT_TUNE_FWD_GROUP > 1).  With B > 1 bands, consecutive thread
+ *                      blocks take different bands (with 8, each XCD's L2 holds one band
+ *                      of the image; more bands give a small image more blocks; batch
+ *                      plans take at most 8), and the workspace grows by B * Ns * Na * Nz
+ *                      elements for the per-band partial sums.
  *   GT_TUNE_BACK_BANDS: angle bands of the backprojection (1, 2, 4, 8), on the same
  *                      pattern (each XCD's L2 holds the padded sinogram rows of one band;
  *                      the workspace grows by B * Nx * Ny * Nz elements).

@@ -84,7 +84,7 @@ def test_tuning_knobs_accept_their_values_and_reject_others(lib):
     from gratopy_amd import autotune
 
     before = autotune.current_variant()
-    valid = {"fwd_group": (1, 2, 4, 8, 16), "back_rows": (1, 2, 4, 8), "fwd_bands": (1, 2, 4, 8),
+    valid = {"fwd_group": (1, 2, 4, 8, 16), "back_rows": (1, 2, 4, 8), "fwd_bands": (1, 2, 4, 8, 16, 32),
              "back_bands": (1, 2, 4, 8), "back_stage": (0, 8, 16), "back_tile": (16, 32),
              "fwd_chunk": (1, 4), "automatic": (0, 1)}
     try:

@@ -1,6 +1,7 @@
 """The automatic launch-shape choice (the library's default) across its
 thresholds: stacks whose geometry, work Nx*Ny*Na*Nz and pixel count put each
-direction on every branch (forward: bands only, banded 16-byte reads, K = 16;
+direction on every branch (forward: bands only, banded 16-byte reads with 8 or
+32 row bands, K = 16;
 backprojection: unstaged with angle bands, LDS-staged with and without angle
 bands, tiles 16 and 32 wide) meet the parity bar slice by slice, and batch
 plans in automatic mode agree with the single-slice path to rounding (each
@@ -36,7 +37,8 @@ def automatic():
 STACKS = [
     ("fan", 128, 90, 192, 16),     # work 2.4e7: bands only / unstaged, angle bands
     ("radon", 128, 120, 192, 8),   # 1.6e7: banded 16-byte reads / unstaged, angle bands
-    ("fan", 256, 360, 384, 4),     # 9.4e7: bands only / staged, angle bands, tiles 32 wide
+    ("fan", 256, 360, 384, 4),     # 9.4e7: banded 16-byte reads / staged, angle bands, 32 wide
+    ("fan", 512, 360, 768, 1),     # 9.4e7, one 512^2 slice: 32 row bands / staged, angle bands
     ("radon", 256, 360, 384, 50),  # 1.2e9: banded 16-byte reads / staged, angle bands, 16 wide
     ("radon", 1024, 60, 1536, 9),  # 5.7e8, 9.4e6 pixels: staged without bands, 16 wide
     ("fan", 512, 720, 768, 85),    # 1.6e10: K = 16 / staged without bands, 32 wide
