@@ -59,6 +59,10 @@ BACK_VARIANTS = tuple(
     {"back_rows": r, "back_bands": 8, "back_stage": 16, "back_tile": t}
     for t in BACK_TILES for r in BACK_ROWS)
 
+# tune() skips forward variants with more than 8 row bands whose band partials
+# would exceed this many bytes of workspace
+MAX_BAND_PARTIAL_BYTES = 4 << 30
+
 _lock = threading.Lock()
 
 
@@ -157,8 +161,13 @@ def tune(projectionsetting, img, sino, repeats: int = 3, store: bool = True) -> 
     from .gratopy import backprojection, forwardprojection
 
     ps = projectionsetting
+    # band partials take bands * Ns * Na * Nz elements of workspace; more than
+    # 8 bands only pay where the launch is small, so large stacks skip them
+    partial = int(sino.numel()) * sino.element_size()
     fwd_times, back_times = {}, {}
     for v in FWD_VARIANTS:
+        if v["fwd_bands"] > 8 and v["fwd_bands"] * partial > MAX_BAND_PARTIAL_BYTES:
+            continue
         set_variant(**v)
         fwd_times[variant_key(v)] = (_time(lambda: forwardprojection(img, ps, sino=sino), repeats), v)
     for v in BACK_VARIANTS:
