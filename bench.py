@@ -26,10 +26,23 @@ sys.path.insert(0, ROOT)
 HBM_PEAK = 8.0e12  # bytes/s, MI355X
 
 WORKLOADS = {
-    # name: (Nz, N, Na, Ns, RE, R, image_width, detector_width, ellipses)
+    # name: (Nz, N, Na, Ns, RE, R, image_width, detector_width, ellipses); RE and R are
+    # None for parallel beam (angles in [0, pi), detector spanning the image diagonal)
     "fanbeam-2048x32": (32, 2048, 1440, 2560, 2048.0, 4096.0, 2048.0, 4096.0, 30),
     "fanbeam-512": (1, 512, 360, 768, 400.0, 800.0, 512.0, 1024.0, 15),
+    "radon-2048x32": (32, 2048, 1440, 2560, None, None, 2048.0, 2048.0 * 2 ** 0.5, 30),
 }
+DEFAULT_WORKLOAD = "fanbeam-2048x32"
+
+
+def projection_kwargs(workload):
+    """(geometry is fanbeam, ProjectionSettings keyword arguments) of a workload."""
+    _, _, _, _, RE, R, iw, dw, _ = WORKLOADS[workload]
+    fan = R is not None
+    kw = dict(image_width=iw, detector_width=dw)
+    if fan:
+        kw.update(R=R, RE=RE)
+    return fan, kw
 
 
 def parse():
@@ -37,7 +50,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=2)
-    p.add_argument("--workload", default="fanbeam-2048x32", choices=sorted(WORKLOADS))
+    p.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-angles", type=int, default=90,
                    help="angles of the bounded CPU-baseline sample (one slice)")
@@ -126,7 +139,9 @@ def bench_tv_stack(ps, img, n_iter, nominal=500):
     diff = max(float((ub[:, :, z] - up[z]).abs().max()) for z in range(Nz))
     rate_b = Nz / (tb / n_iter * nominal)
     rate_p = Nz / (tp / n_iter * nominal)
-    return {"metric": "TV(500) slices/s, 32 x 2048^2 fanbeam stack", "batched_graph": round(rate_b, 4),
+    Nx, Ny, Nz = (int(n) for n in img.shape)
+    stack = f"{Nz} x {Nx}^2 {'fanbeam' if ps.is_fan else 'radon'} stack"
+    return {"metric": f"TV(500) slices/s, {stack}", "batched_graph": round(rate_b, 4),
             "plain_per_slice": round(rate_p, 4), "speedup": round(rate_b / rate_p, 3),
             "iterations_timed": n_iter, "ms_per_iteration_batched": round(tb / n_iter * 1e3, 2),
             "ms_per_iteration_plain": round(tp / n_iter * 1e3, 2),
@@ -255,21 +270,29 @@ def make_stack(Nz, N, n_ellipses, dtype=np.float32):
     return stack
 
 
-def cpu_baseline(ps_kwargs, N, n_angles, n_det, slice_img):
+def cpu_baseline(fan, ps_kwargs, N, n_angles, n_det, slice_img):
     """Oracle (C restatement of the reference kernels, OpenMP) on one slice
     with a subset of the angles; returns pixel-angle updates per second."""
     import gratopy_amd as gratopy
     import oracle
 
-    ps = gratopy.ProjectionSettings(None, gratopy.FANBEAM, (N, N), n_angles, n_det, **ps_kwargs)
+    geometry = gratopy.FANBEAM if fan else gratopy.RADON
+    ps = gratopy.ProjectionSettings(None, geometry, (N, N), n_angles, n_det, **ps_kwargs)
     t = ps.struct
     dt = np.dtype("float32")
     img = np.asfortranarray(slice_img)
     oracle.lib()
     t0 = time.perf_counter()
-    sino = oracle.fan_fwd(img, t["ofs_dict"][dt], t["sdpd_dict"][dt], t["geo_dict"][dt], n_det, n_angles)
+    if fan:
+        sino = oracle.fan_fwd(img, t["ofs_dict"][dt], t["sdpd_dict"][dt], t["geo_dict"][dt], n_det,
+                              n_angles)
+    else:
+        sino = oracle.radon_fwd(img, t["ofs_dict"][dt], t["geo_dict"][dt], n_det, n_angles)
     t1 = time.perf_counter()
-    oracle.fan_back(sino, t["ofs_dict"][dt], t["sdpd_dict"][dt], t["geo_dict"][dt], (N, N))
+    if fan:
+        oracle.fan_back(sino, t["ofs_dict"][dt], t["sdpd_dict"][dt], t["geo_dict"][dt], (N, N))
+    else:
+        oracle.radon_back(sino, t["ofs_dict"][dt], t["geo_dict"][dt], (N, N))
     t2 = time.perf_counter()
     work = N * N * n_angles
     return work / (t2 - t0), (t1 - t0), (t2 - t1)
@@ -313,9 +336,10 @@ def main():
         print(json.dumps(out), file=json_out, flush=True)
         return
     Nz, N, Na, Ns, RE, R, iw, dw, n_ell = WORKLOADS[args.workload]
-    ps_kwargs = dict(R=R, RE=RE, image_width=iw, detector_width=dw)
+    fan, ps_kwargs = projection_kwargs(args.workload)
     device = "cuda:0"
-    ps = gratopy.ProjectionSettings(device, gratopy.FANBEAM, (N, N), Na, Ns, **ps_kwargs)
+    geometry = gratopy.FANBEAM if fan else gratopy.RADON
+    ps = gratopy.ProjectionSettings(device, geometry, (N, N), Na, Ns, **ps_kwargs)
 
     host = make_stack(Nz, N, n_ell)
     img = dev.to_device(host, device, order="F")
@@ -367,14 +391,15 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         n_ang = min(args.cpu_angles, Na)
-        rate, tf, tb = cpu_baseline(ps_kwargs, N, n_ang, Ns, host[:, :, 0])
+        rate, tf, tb = cpu_baseline(fan, ps_kwargs, N, n_ang, Ns, host[:, :, 0])
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         cpu = {"value": rate / 1e9, "unit": "GUPS", "cores": threads, "kind": "port",
-               "sample": f"oracle fan fwd+back, 1 slice {N}x{N}, {n_ang} of {Na} angles, {Ns} det "
-                         f"(fwd {tf:.2f}s, back {tb:.2f}s)"}
+               "sample": f"oracle {'fan' if fan else 'radon'} fwd+back, 1 slice {N}x{N}, "
+                         f"{n_ang} of {Na} angles, {Ns} det (fwd {tf:.2f}s, back {tb:.2f}s)"}
 
     out = {
-        "metric": "fwd+back GUPS (pixel*angle/s) fanbeam 2048^2",
+        "metric": ("fwd+back GUPS (pixel*angle/s) fanbeam 2048^2" if args.workload == DEFAULT_WORKLOAD
+                   else f"fwd+back GUPS (pixel*angle/s) {'fanbeam' if fan else 'radon'} {N}^2 x {Nz}"),
         "value": round(gups, 3),
         "unit": "GUPS",
         "n_gpus": 1,
