@@ -364,7 +364,9 @@ static FwdShape fwd_shape(const Layout &L) {
     return {t.fwd_group, g ? t.fwd_bands : 1, g ? t.fwd_chunk : 1};
   }
   const double w = work_of(L);
-  if (L.fan && w < 5e7) return {4, 8, 1};
+  // tiny calls: no row-padded copies (one image copy fewer to launch); the
+  // row bands stay, without them a tiny launch has a handful of blocks
+  if (w < 2e6 || (L.fan && w < 5e7)) return {4, 8, 1};
   const int K = w < 2e9 ? 8 : 16;
   // 32 row bands where 8 would launch fewer than 4096 blocks (a single slice
   // leaves the GPU part idle), with at least 16 rows per band
@@ -375,6 +377,7 @@ static FwdShape fwd_shape(const Layout &L) {
 static BackShape back_shape(const Layout &L) {
   const Tuning &t = tuning();
   if (!t.automatic) return {t.back_rows, t.back_bands, t.back_stage, t.back_tile};
+  if (work_of(L) < 2e6) return {4, 1, 0, 16};  // launch-bound: no band reduction
   if (work_of(L) < 5e7) return {4, 8, 0, 16};
   // Staged tiles hold 2048 pixels at either width; angle bands while the
   // tiles of the call fill the GPU fewer than ~3 times (4096 tiles).

@@ -113,14 +113,19 @@ def test_automatic_workspace_follows_the_automatic_shape(lib):
     """In automatic mode the workspace query sizes what the chosen shape uses:
     a small parallel-beam forward reads row-padded copies and a small fanbeam
     one does not; the backprojection keeps angle-band partials only below
-    4096 * 2048 pixels."""
+    4096 * 2048 pixels; tiny calls (below 2e6 pixel-angles) use no padded
+    copies and no backprojection bands."""
     from gratopy_amd import autotune
 
     before = autotune.current_variant()
     try:
         autotune.set_variant(automatic=1)
         ws = lib.gt_projection_workspace_bytes
-        es, N, Ns, Na = 4, 100, 150, 30
+        es, N, Ns, Na = 4, 100, 150, 300
+        tiny = 30  # angles: 3e5 pixel-angles
+        # F order: one y-fastest copy and the forward's 8 band partials
+        assert ws(0, 0, 0, N, N, Ns, tiny, 1, 0) < es * ((N + 4) * N + 8 * Ns * tiny)
+        assert ws(0, 1, 0, N, N, Ns, tiny, 1, 0) < es * N * N  # no band partials
         radon_f, fan_f = ws(0, 0, 0, N, N, Ns, Na, 1, 0), ws(1, 0, 0, N, N, Ns, Na, 1, 0)
         assert radon_f >= es * ((N + 4) * N * 2 + 8 * Ns * Na)
         assert fan_f < es * (N + 4) * N * 2 + es * 8 * Ns * Na

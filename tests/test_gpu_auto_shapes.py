@@ -1,8 +1,8 @@
 """The automatic launch-shape choice (the library's default) across its
 thresholds: stacks whose geometry, work Nx*Ny*Na*Nz and pixel count put each
 direction on every branch (forward: bands only, banded 16-byte reads with 8 or
-32 row bands, K = 16;
-backprojection: unstaged with angle bands, LDS-staged with and without angle
+32 row bands, K = 16; backprojection: unstaged without (tiny calls) and with
+angle bands, LDS-staged with and without angle
 bands, tiles 16 and 32 wide) meet the parity bar slice by slice, and batch
 plans in automatic mode agree with the single-slice path to rounding (each
 picks its own shape, so they are not bitwise equal)."""
@@ -35,6 +35,7 @@ def automatic():
 
 # (geometry, N, angles, detectors, slices) -> forward / backprojection shape
 STACKS = [
+    ("radon", 64, 30, 90, 1),      # 1.2e5: bands, one-pixel reads / unstaged, no bands
     ("fan", 128, 90, 192, 16),     # work 2.4e7: bands only / unstaged, angle bands
     ("radon", 128, 120, 192, 8),   # 1.6e7: banded 16-byte reads / unstaged, angle bands
     ("fan", 256, 360, 384, 4),     # 9.4e7: banded 16-byte reads / staged, angle bands, 32 wide
