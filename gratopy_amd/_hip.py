@@ -48,6 +48,8 @@ PROTOTYPES: dict[str, tuple] = {
     "gt_radon_back": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _vp, _sz]),
     "gt_fan_fwd": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _vp, _sz]),
     "gt_fan_back": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _vp, _sz]),
+    "gt_projection_noalloc": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i,
+                                   _i, _i]),
     "gt_single_line": (_i, [_vp, _i, _i, _vp, _i, _i, _vp, _vp, _vp, _i, _i, _i, _i, _i]),
     "gt_weight_sino": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _i, _i, _i]),
     "gt_axpy": (_i, [_vp, _i, _vp, _d, _vp, _sz]),

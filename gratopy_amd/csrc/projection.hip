@@ -13,6 +13,8 @@
 // (gratopy/radon.cl:183-241, gratopy/fanbeam.cl:290-391).
 
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 #include "kernels.h"
 
@@ -664,6 +666,89 @@ extern "C" int gt_fan_back(gt_stream_t stream, int dtype, void *img, const void 
       (back_impl<double, true>(st, (double *)img, (const double *)sino, (const double *)ofs,
                                (const double *)sdpd, (const double *)geo, L, img_order,
                                sino_order, workspace, workspace_bytes)));
+}
+
+// ---------------------------------------------------------------------------
+// Projections without allocation: the workspace is a device array of the
+// library (one per GPU, held until the process ends).  The array's name is
+// a host-side stand-in in host code; its device address on each GPU comes
+// from hipGetSymbolAddress with that GPU current, once, and is kept here.
+// ---------------------------------------------------------------------------
+namespace gt {
+
+static __device__ __align__(256) unsigned char scratch[GT_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];
+
+// Device address of `scratch` on the stream's device, which must be current.
+static int scratch_address(hipStream_t st, void **out) {
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess) return fail(GT_EHIP, "hipGetDevice failed");
+  hipDevice_t sdev = cur;
+  if (st != nullptr && hipStreamGetDevice(st, &sdev) != hipSuccess)
+    return fail(GT_EHIP, "hipStreamGetDevice failed");
+  if (sdev != cur)
+    return fail(GT_EINVAL, "the stream belongs to device " + std::to_string(sdev) + " but device " +
+                               std::to_string(cur) + " is current");
+  GT_REQUIRE(cur >= 0 && cur < kMaxDevices, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[cur] == nullptr) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      return fail(GT_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[cur] = p;
+  }
+  *out = g_scratch_ptr[cur];
+  return GT_OK;
+}
+
+// One projection (any geometry and direction) on a given workspace.
+template <typename real>
+static int project_with(hipStream_t st, bool fan, int adjoint, void *out, const void *in,
+                        const void *ofs, const void *sdpd, const void *geo, const Layout &L,
+                        int out_order, int in_order, void *ws, size_t wb) {
+  real *o = static_cast<real *>(out);
+  const real *i = static_cast<const real *>(in), *t = static_cast<const real *>(ofs);
+  const real *d = static_cast<const real *>(sdpd), *g = static_cast<const real *>(geo);
+  if (adjoint)
+    return fan ? back_impl<real, true>(st, o, i, t, d, g, L, out_order, in_order, ws, wb)
+               : back_impl<real, false>(st, o, i, t, nullptr, g, L, out_order, in_order, ws, wb);
+  return fan ? fwd_impl<real, true>(st, o, i, t, d, g, L, out_order, in_order, ws, wb)
+             : fwd_impl<real, false>(st, o, i, t, nullptr, g, L, out_order, in_order, ws, wb);
+}
+
+}  // namespace gt
+
+extern "C" int gt_projection_noalloc(gt_stream_t stream, int geometry, int adjoint, int dtype,
+                                     void *out, const void *in, const void *ofs, const void *sdpd,
+                                     const void *geo, int Nx, int Ny, int Ns, int Na, int Nz,
+                                     int out_order, int in_order) {
+  GT_REQUIRE(geometry == GT_GEOMETRY_RADON || geometry == GT_GEOMETRY_FAN, "unknown geometry");
+  GT_REQUIRE(adjoint == 0 || adjoint == 1, "adjoint must be 0 or 1");
+  int rc = check_common(dtype, out, in, ofs, geo, out_order, in_order);
+  if (rc) return rc;
+  const bool fan = geometry == GT_GEOMETRY_FAN;
+  GT_REQUIRE(!fan || sdpd != nullptr, "NULL sdpd");
+  const Layout L{Nx, Ny, Ns, Na, Nz, fan};
+  if ((rc = check_sizes(L))) return rc;
+  const size_t need = adjoint ? back_ws(dtype, L) : fwd_ws(dtype, L, in_order);
+  if (need > GT_NOALLOC_SCRATCH_BYTES)
+    return fail(GT_ENOSPC, "projection needs " + std::to_string(need) +
+                               " bytes of workspace; the no-allocation entry point holds " +
+                               std::to_string((size_t)GT_NOALLOC_SCRATCH_BYTES) +
+                               " (use the entry points with a caller workspace)");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  void *ws = nullptr;
+  if ((rc = scratch_address(st, &ws))) return rc;
+  const size_t wb = GT_NOALLOC_SCRATCH_BYTES;
+  if (dtype == GT_FLOAT64)
+    return project_with<double>(st, fan, adjoint, out, in, ofs, sdpd, geo, L, out_order, in_order,
+                                ws, wb);
+  return project_with<float>(st, fan, adjoint, out, in, ofs, sdpd, geo, L, out_order, in_order, ws,
+                             wb);
 }
 
 extern "C" int gt_single_line(gt_stream_t stream, int geometry, int dtype, void *sino, int x,

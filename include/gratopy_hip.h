@@ -125,6 +125,23 @@ int gt_fan_back(gt_stream_t stream, int dtype, void *img, const void *sino, cons
                 const void *sdpd, const void *geo, int Nx, int Ny, int Ns, int Na, int Nz,
                 int img_order, int sino_order, void *workspace, size_t workspace_bytes);
 
+/* Projection without allocation: gt_{radon,fan}_{fwd,back} (geometry GT_GEOMETRY_RADON
+ * or GT_GEOMETRY_FAN; adjoint 0 = forward, out = sino, in = img; adjoint 1 =
+ * backprojection, out = img, in = sino; sdpd may be NULL for Radon), with the
+ * workspace taken from a device array that the library holds on every GPU it runs
+ * on, GT_NOALLOC_SCRATCH_BYTES (128 MiB) each, until the process ends.  No
+ * counterpart in the reference: for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * Allocates nothing.  The stream must belong to the current device.  A call whose
+ * workspace (gt_projection_workspace_bytes) exceeds the array returns GT_ENOSPC.
+ * Every call on one GPU uses the same array: calls that overlap on one GPU (on
+ * different streams, or from different threads) race.  Results are those of the
+ * entry points above. */
+#define GT_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int gt_projection_noalloc(gt_stream_t stream, int geometry, int adjoint, int dtype, void *out,
+                          const void *in, const void *ofs, const void *sdpd, const void *geo,
+                          int Nx, int Ny, int Ns, int Na, int Nz, int out_order, int in_order);
+
 /* Sinogram of a unit impulse at pixel (x, y): one column of the system matrix.
  * sdpd is ignored (may be NULL) for the parallel geometry.
  * Replaces single_line_radon_* (gratopy/radon.cl:263-341) and

@@ -176,3 +176,15 @@ def test_batch_arguments_rejected_without_a_device(lib):
     assert lib.gt_batch_launch(None, None, 0, 0.0) == _hip.GT_EINVAL
     assert lib.gt_batch_plan_destroy(None) == _hip.GT_OK
     assert lib.gt_batch_plan_bytes(None) == 0
+
+
+def test_noalloc_rejects_calls_beyond_its_array(lib):
+    # a 4096^2 x 8 float64 fan backprojection needs ~0.5 GiB of workspace; the
+    # no-allocation entry point holds 128 MiB and refuses before any HIP call
+    need = lib.gt_projection_workspace_bytes(1, 1, 1, 4096, 4096, 4096, 2048, 8, 0)
+    assert need > (128 << 20)
+    p = [ctypes.c_void_p(16 * k) for k in range(1, 6)]
+    rc = lib.gt_projection_noalloc(None, 1, 1, 1, *p, 4096, 4096, 4096, 2048, 8, 0, 0)
+    assert rc == _hip.GT_ENOSPC and "no-allocation" in _hip.last_error()
+    rc = lib.gt_projection_noalloc(None, 1, 2, 1, *p, 64, 64, 64, 64, 1, 0, 0)
+    assert rc == _hip.GT_EINVAL and "adjoint" in _hip.last_error()
