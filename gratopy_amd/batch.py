@@ -46,7 +46,11 @@ class BatchPlan:
         n = len(settings)
         if not (n and len(imgs) == n and len(sinos) == n):
             raise ValueError("settings, imgs and sinos must be non-empty lists of equal length")
+        for name, extra in (("datas", datas), ("imgs2", imgs2), ("fields", fields), ("params", params)):
+            if extra is not None and len(extra) != n:
+                raise ValueError(f"{name} must have one entry per slice")
         dtype = dev.to_np_dtype(imgs[0].dtype)
+        device = imgs[0].device
         self.dtype = dtype
         self.n = n
         self.settings = list(settings)
@@ -54,11 +58,40 @@ class BatchPlan:
         items = (_hip.BatchItem * n)()
         for i, ps in enumerate(settings):
             img, sino = _f_order(imgs[i], "img"), _f_order(sinos[i], "sino")
-            if dev.to_np_dtype(img.dtype) != dtype or dev.to_np_dtype(sino.dtype) != dtype:
-                raise ValueError("all slices of a batch must have one dtype")
-            if tuple(img.shape) != tuple(ps.img_shape) or tuple(sino.shape) != tuple(ps.sinogram_shape):
-                raise ValueError(f"slice {i}: shapes do not match its ProjectionSettings")
+            # every pointer handed to the kernels must be a tensor of this
+            # batch's dtype, on its HIP device, covering what the slice's
+            # geometry reads or writes
+            operands = [("img", img, ps.img_shape), ("sino", sino, ps.sinogram_shape)]
+            if datas is not None:
+                operands.append(("data", _f_order(datas[i], "data"), ps.sinogram_shape))
+            if imgs2 is not None:
+                operands.append(("img2", _f_order(imgs2[i], "img2"), ps.img_shape))
+            for name, t, shape in operands:
+                if dev.to_np_dtype(t.dtype) != dtype:
+                    raise ValueError("all slices of a batch must have one dtype")
+                if tuple(t.shape) != tuple(shape):
+                    raise ValueError(f"slice {i}: {name} shapes do not match its ProjectionSettings")
+            if fields is not None:
+                fld = fields[i]
+                if dev.to_np_dtype(fld.dtype) != dtype or not fld.is_contiguous():
+                    raise ValueError(f"slice {i}: field must be a contiguous tensor of the batch dtype")
+                if fld.numel() < 4 * ps.img_shape[0] * ps.img_shape[1]:
+                    raise ValueError(f"slice {i}: field needs 4 * Nx * Ny entries")
+                operands.append(("field", fld, None))
+            for name, t, _ in operands:
+                if t.device.type != "cuda" or t.device != device:
+                    raise ValueError(f"slice {i}: {name} must live on the batch's HIP device "
+                                     f"({device}), got {t.device}")
+            if ps.queue.device is not None and dev.torch.device(ps.queue.device) != device:
+                raise ValueError(f"slice {i}: its ProjectionSettings live on {ps.queue.device}, "
+                                 f"the batch on {device}")
             ps.ensure_dtype(dtype)
+            tables = [ps.ofs_buf[dtype], ps.geometry_information[dtype]]
+            if ps.is_fan:
+                tables.append(ps.sdpd_buf[dtype])
+            # the plan holds raw pointers to the tables: keep these tensors
+            # alive even if the settings later replace them (set_angle_weights)
+            self._keep.append(tables)
             it = items[i]
             it.geometry = _hip.GEOM_FAN if ps.is_fan else _hip.GEOM_RADON
             it.Nx, it.Ny = ps.img_shape
@@ -69,9 +102,9 @@ class BatchPlan:
             it.img = img.data_ptr()
             it.sino = sino.data_ptr()
             if datas is not None:
-                it.data = _f_order(datas[i], "data").data_ptr()
+                it.data = datas[i].data_ptr()
             if imgs2 is not None:
-                it.img2 = _f_order(imgs2[i], "img2").data_ptr()
+                it.img2 = imgs2[i].data_ptr()
             if fields is not None:
                 it.field = fields[i].data_ptr()
             if params is not None:

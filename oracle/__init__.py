@@ -32,12 +32,46 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 _lib = None
 
 
+FAST_PATHS = {v: os.path.join(HERE, f"liboracle_fast_{v}.so") for v in ("v3", "v4")}
+
+
 def build(force: bool = False) -> str:
-    """Compile liboracle.so with the committed Makefile (gcc, OpenMP)."""
-    if force or not os.path.exists(LIB_PATH):
-        subprocess.run(["make", "-B" if force else "-s", "-C", HERE, "liboracle.so"], check=True,
+    """Compile liboracle.so (the checker) and the two CPU-baseline timing
+    builds with the committed Makefile (gcc, OpenMP)."""
+    if force or not all(os.path.exists(p) for p in (LIB_PATH, *FAST_PATHS.values())):
+        subprocess.run(["make", "-B" if force else "-s", "-C", HERE, "all"], check=True,
                        stdout=subprocess.DEVNULL)
     return LIB_PATH
+
+
+def _has_avx512() -> bool:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            return any(line.startswith("flags") and " avx512f" in line for line in fh)
+    except OSError:
+        return False
+
+
+class timing_build:
+    """Context manager: the oracle functions run on the CPU-baseline timing
+    build (FMA contraction allowed; AVX-512 where the host has it) instead of
+    the checker.  Only bench.py's CPU baseline uses it."""
+
+    def __enter__(self):
+        global _lib
+        variant = "v4" if _has_avx512() else "v3"
+        path = FAST_PATHS[variant]
+        if not os.path.exists(path):
+            build()
+        self._saved = _lib
+        _lib = ctypes.CDLL(path)
+        self.variant = variant
+        return self
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self._saved
+        return False
 
 
 def lib() -> ctypes.CDLL:

@@ -45,3 +45,22 @@ def test_shapes_and_dtypes_must_match_settings():
         BatchPlan([ps], [_img((8, 9))], [_img((12, 5))])
     with pytest.raises(ValueError, match="one dtype"):
         BatchPlan([ps], [_img((8, 8))], [_img((12, 5), np.float64)])
+
+
+def test_every_operand_is_checked_before_the_library_is_called():
+    ps = _ps()
+    img, sino = _img((8, 8)), _img((12, 5))
+    # well-formed but on the CPU: rejected, no host pointer reaches a kernel
+    with pytest.raises(ValueError, match="HIP device"):
+        BatchPlan([ps], [img], [sino])
+    # measured sinogram with fewer angles than the settings
+    with pytest.raises(ValueError, match="data shapes do not match"):
+        BatchPlan([ps], [img], [sino], datas=[_img((12, 4))])
+    # a float64 second image in a float32 batch
+    with pytest.raises(ValueError, match="one dtype"):
+        BatchPlan([ps], [img], [sino], imgs2=[_img((8, 8), np.float64)])
+    # TV dual variable too small for the slice
+    with pytest.raises(ValueError, match="4 \\* Nx \\* Ny"):
+        BatchPlan([ps], [img], [sino], fields=[torch.zeros(4 * 8 * 8 - 1)])
+    with pytest.raises(ValueError, match="one entry per slice"):
+        BatchPlan([ps], [img], [sino], datas=[])
