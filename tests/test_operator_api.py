@@ -173,3 +173,41 @@ def test_fanbeam_operator_shapes():
     assert F.angles.half_circle is False
     G = Fanbeam(600.0, 32, 10)
     assert G.state["source_distances"] == (600.0, 300.0)
+
+
+class _Matrix(Operator):
+    """A leaf applying a fixed matrix (host numpy), for the algebra's semantics."""
+
+    def __init__(self, name, M):
+        super().__init__(name=name, input_shape=(M.shape[1],), output_shape=(M.shape[0],))
+        self.M = M
+
+    def apply_to(self, argument, output=None, **kwargs):
+        result = self.scalar * (self.M @ argument)
+        if output is None:
+            return result
+        output[...] = result
+        return output
+
+
+def test_algebra_applies_as_linear_maps():
+    rng = np.random.default_rng(0)
+    Ma, Mb, Mc = rng.standard_normal((4, 3)), rng.standard_normal((4, 3)), rng.standard_normal((3, 3))
+    A, B, C = _Matrix("A", Ma), _Matrix("B", Mb), _Matrix("C", Mc)
+    x = rng.standard_normal(3)
+    np.testing.assert_allclose((2 * A - 3 * B) * x, 2 * Ma @ x - 3 * Mb @ x)
+    np.testing.assert_allclose(((A + B) * C) * x, (Ma + Mb) @ (Mc @ x))
+    np.testing.assert_allclose((4 * (A * C * C)) * x, 4 * Ma @ Mc @ Mc @ x)
+    out = np.zeros(4)
+    assert (A * C).apply_to(x, output=out) is out
+    np.testing.assert_allclose(out, Ma @ Mc @ x)
+    # factors of composed leaves move to the leftmost factor; the operands stay unchanged
+    P = A * (2 * C)
+    assert repr(P) == "2*A*C" and C.scalar == 1 and A.scalar == 1
+    P.scalar = 3  # a composition multiplies its leftmost factor
+    assert repr(P) == "6*A*C"
+    S = A + B
+    S.scalar = -1  # a sum multiplies every term
+    assert repr(S) == "(-1)*A + (-1)*B" and S.scalar == 1
+    assert ZERO * A == ZERO and A * ZERO == ZERO
+    np.testing.assert_allclose((A + ZERO) * x, Ma @ x)
