@@ -2,11 +2,11 @@
 
 By default every call picks its launch shape from its size (``automatic``).
 Knobs (the C ABI's ``GT_TUNE_*`` keys): the forward kernels assign K
-adjacent detector bins to a lane (``fwd_group`` K in 1, 2, 4, 8, 16; K = 1 is the
-per-ray kernel), optionally split the strip walk into 8 or 32 row bands
+adjacent detector bins to a lane (``fwd_group`` K in 1, 2, 4, 8, 16, 32; K = 1 is
+the per-ray kernel), optionally split the strip walk into 8 or 32 row bands
 (``fwd_bands``; with 8, one band per XCD; 32 give small images four times the
-blocks; partial sums added in band order) and read strip rows 16 bytes per
-load (``fwd_chunk`` = 4).  The backprojection assigns
+blocks; partial sums added in band order), read strip rows 16 bytes per
+load (``fwd_chunk`` = 4; always with K = 32).  The backprojection assigns
 RY pixels along y to a lane (``back_rows`` in 1, 2, 4, 8), sums all angles per
 tile or splits them into 8 angle bands (``back_bands``), and reads its taps
 from memory or stages each tile's sinogram windows in LDS for chunks of 8 or
@@ -45,10 +45,12 @@ FWD_BANDS = (1, 8, 32)
 FWD_CHUNKS = (1, 4)
 BACK_STAGES = (0, 8, 16)
 BACK_TILES = (16, 32)
-# forward variants; bands and chunked reads need bin groups
+# forward variants; bands and chunked reads need bin groups; 32 bins per lane
+# come with chunked reads only
 FWD_VARIANTS = ({"fwd_group": 1, "fwd_bands": 1, "fwd_chunk": 1},) + tuple(
     {"fwd_group": k, "fwd_bands": b, "fwd_chunk": c}
-    for k in FWD_GROUPS if k > 1 for b in FWD_BANDS for c in FWD_CHUNKS)
+    for k in FWD_GROUPS if k > 1 for b in FWD_BANDS for c in FWD_CHUNKS) + tuple(
+    {"fwd_group": 32, "fwd_bands": b, "fwd_chunk": 4} for b in FWD_BANDS)
 # backprojection variants: unstaged (with or without angle bands), staged, and
 # staged in chunks of 16 angles over 8 angle bands
 BACK_VARIANTS = tuple(
@@ -207,7 +209,8 @@ def tune_batch(settings, imgs, sinos, repeats: int = 2) -> dict:
 
     fwd_times, back_times = {}, {}
     for v in FWD_VARIANTS:
-        if v["fwd_bands"] > 8:  # batch plans take at most 8 row bands
+        # batch plans take at most 8 row bands and 16 bins per lane
+        if v["fwd_bands"] > 8 or v["fwd_group"] > 16:
             continue
         set_variant(**v)
         fwd_times[variant_key(v)] = (timed(_hip.BATCH_FORWARD), v)

@@ -584,6 +584,7 @@ extern "C" int gt_batch_plan_create(gt_batch_plan **plan, int dtype, int n_items
   // (128^2..1024^2, 60..720 angles); batches have no backprojection angle bands
   Tuning t = tuning();
   t.fwd_bands = std::min(t.fwd_bands, 8);  // batch forwards take up to 8 row bands
+  t.fwd_group = std::min(t.fwd_group, 16);  // and up to 16 bins per lane
   if (t.automatic) {
     t.fwd_group = 16;
     t.fwd_bands = 8;

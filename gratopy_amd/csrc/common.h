@@ -23,7 +23,7 @@ inline size_t align_up(size_t n, size_t a = 256) { return (n + a - 1) / a * a; }
 
 // ---- launch-shape knobs set by the autotuner (gt_tune_set) -----------------
 struct Tuning {
-  int fwd_group = 4;  // detector bins per lane in the forward kernels (1, 2, 4, 8, 16)
+  int fwd_group = 4;  // detector bins per lane in the forward kernels (1, 2, 4, 8, 16, 32)
   int back_rows = 1;  // pixels per lane (along y) in the backprojection (1, 2, 4, 8)
   int fwd_bands = 1;  // row bands of the forward strip walk (1, 2, 4, 8, 16, 32; fwd_group > 1)
   int back_bands = 1;  // angle bands of the backprojection (1, 2, 4, 8)

@@ -71,6 +71,19 @@ struct Slots {
   static constexpr int kCount = K + 4;
 };
 
+// One strip pixel's contribution to its lane's slots (rows of 256 entries,
+// one entry per lane; bin s0 + k is slot k + 2): value v at detector
+// position p in slot units, clamped to [0.5, K + 2.5] so that truncation is
+// floor; hat weights slot[floor p] += (1 - f) v, slot[floor p + 1] += f v,
+// f = p - floor p.
+template <typename real>
+__device__ __forceinline__ void deposit(real *acc, real p, real v) {
+  real *slot = acc + (int)p * 256;
+  const real f = Arith<real>::fract(p);
+  slot[0] += v - f * v;
+  slot[256] += f * v;
+}
+
 // ---------------------------------------------------------------------------
 // epilogues
 // ---------------------------------------------------------------------------
@@ -546,11 +559,7 @@ __device__ __forceinline__ void fan_fwd_group(const Job<real> &J, int s0, int a,
       for (int i = 0; i < CH; ++i) {
         const real r = A::rcp(t);
         const real p = A::clamp(q * r, lo, hi);
-        const real f = A::fract(p);
-        const real v = c.v[i] * r;
-        real *slot = acc + (int)p * 256;
-        slot[0] += v - f * v;
-        slot[256] += f * v;
+        deposit(acc, p, c.v[i] * r);
         t += F.ddx;
         q += dq;
       }
@@ -560,10 +569,11 @@ __device__ __forceinline__ void fan_fwd_group(const Job<real> &J, int s0, int a,
   for (int k = 0; k < K; ++k) {
     const int s = s0 + k;
     if (s >= J.Ns) continue;
+    const real sum = acc[(k + 2) * 256];
     if (BANDED)
-      *band_slot(J, B, s, a, z) = acc[(k + 2) * 256];
+      *band_slot(J, B, s, a, z) = sum;
     else
-      fan_store(J, s, a, z, acc[(k + 2) * 256]);
+      fan_store(J, s, a, z, sum);
   }
 }
 
@@ -631,12 +641,7 @@ __device__ __forceinline__ void radon_fwd_group(const Job<real> &J, int s0, int 
       const RowChunk<real, CH> c = load_chunk<real, CH>(px);
 #pragma unroll
       for (int i = 0; i < CH; ++i) {
-        const real v = c.v[i];
-        const real p = A::clamp(pr, lo, hi);
-        const real f = A::fract(p);
-        real *slot = acc + (int)p * 256;
-        slot[0] += v - f * v;
-        slot[256] += f * v;
+        deposit(acc, A::clamp(pr, lo, hi), c.v[i]);
         pr += X;
       }
     }
@@ -646,10 +651,11 @@ __device__ __forceinline__ void radon_fwd_group(const Job<real> &J, int s0, int 
   for (int k = 0; k < K; ++k) {
     const int s = s0 + k;
     if (s >= J.Ns) continue;
+    const real sum = acc[(k + 2) * 256];
     if (BANDED)
-      *band_slot(J, B, s, a, z) = acc[(k + 2) * 256];
+      *band_slot(J, B, s, a, z) = sum;
     else
-      store_fwd(J, s, a, z, acc[(k + 2) * 256] * scale);
+      store_fwd(J, s, a, z, sum * scale);
   }
 }
 
@@ -847,17 +853,25 @@ __device__ __forceinline__ void band_back_reduce_entry(const Job<real> &J,
 constexpr int kStageWidth = 320;  // LDS elements per staged window
 
 // Window of one angle for one tile: padded-sinogram indices [w0, w0 + n),
-// zero outside [0, Ns + 4).  Coordinates are relative to w0: radon
-// q = c0 x + c1 y + k0 (k0 = c2 - w0); fan t = c0 x + c1 y + k0,
-// ss = c3 x + c4 y + k1, p = ss / t + k2 (k2 = mid_det + 2 - w0).  Every
-// pixel of the tile has 1 <= p < n - 2 (to rounding), so the taps floor(p),
-// floor(p) + 1 need no clamp: where the unstaged kernel clamps p into
-// [0.5, Ns + 2.5], both windows hold zeros.  n = 0 marks an angle read from
-// memory instead (window wider than kStageWidth, or a fan tile reaching
-// t <= 0); it keeps unshifted constants and the clamp [lo, hi].
+// zero outside [0, Ns + 4).  Coordinates are taken relative to the tile's
+// corner pixel (xa, ya), with the corner's values formed in double: for a
+// lane at (xa + dx, ya + dy)
+//   radon  p = c0 dx + c1 dy + k0              (k0 = p(xa, ya) - w0)
+//   fan    t = c0 dx + c1 dy + k0,  q = k3 dx + k4 dy + k1,  p = q / t + k2
+// where, for a staged window, q = ss + (mid_det + 2 - w0) t is the window
+// coordinate times t (k0 = t(xa, ya), k1 = q(xa, ya), k3 = c3 + (mid_det + 2
+// - w0) c0, k4 likewise, k2 = 0).  Every term is then of the size of the
+// window (a few hundred at most) instead of the detector (up to Ns), so
+// single precision keeps ~2^-24 of the window, not of Ns.  Every pixel of
+// the tile has 1 <= p < n - 2 (to rounding), so the taps floor(p), floor(p)
+// + 1 need no clamp: where the unstaged kernel clamps p into [0.5, Ns + 2.5],
+// both windows hold zeros.  n = 0 marks an angle read from memory instead
+// (window wider than kStageWidth, or a fan tile reaching t <= 0); its
+// constants give p in padded-sinogram indices (fan: q = ss, k2 = mid_det +
+// 2), clamped to [lo, hi].
 template <typename real>
 struct StageWindow {
-  real k0, k1, k2, lo, hi;
+  real k0, k1, k2, k3, k4, lo, hi;
   int w0, n;
 };
 
@@ -892,32 +906,44 @@ __device__ __forceinline__ StageWindow<real> stage_window(const Job<real> &J, in
   const int w0 = ok ? (int)floor(pmin) - 1 : 0;
   W.w0 = w0;
   W.n = ok ? (int)floor(pmax) + 3 - w0 : 0;
-  W.k0 = FAN ? c.c[2] : (real)((double)c.c[2] - w0);
-  W.k1 = FAN ? c.c[5] : (real)0;
-  W.k2 = (real)(mid - w0);
+  const double xc = xa, yc = ya;
+  if (FAN) {
+    const double tc = (double)c.c[0] * xc + (double)c.c[1] * yc + (double)c.c[2];
+    const double sc = (double)c.c[3] * xc + (double)c.c[4] * yc + (double)c.c[5];
+    const double k = ok ? mid - w0 : 0.0;  // window shift folded into q
+    W.k0 = (real)tc;
+    W.k1 = (real)(sc + k * tc);
+    W.k3 = (real)((double)c.c[3] + k * (double)c.c[0]);
+    W.k4 = (real)((double)c.c[4] + k * (double)c.c[1]);
+    W.k2 = (real)(ok ? 0.0 : mid);
+  } else {
+    const double pc = (double)c.c[0] * xc + (double)c.c[1] * yc + (double)c.c[2];
+    W.k0 = (real)(ok ? pc - w0 : pc);
+    W.k1 = W.k2 = W.k3 = W.k4 = (real)0;
+  }
   W.lo = (real)0.5;
   W.hi = (real)J.Ns + (real)2.5;
   return W;
 }
 
-// Adds one angle's contribution to the RY pixels (x, y0 + r) of this lane.
-// STAGED: `row` is the angle's LDS window (no clamp); otherwise the padded
-// sinogram row in memory (clamped as in the unstaged kernel).
+// Adds one angle's contribution to the RY pixels (xa + dx, ya + dy + r) of
+// this lane.  STAGED: `row` is the angle's LDS window (no clamp); otherwise
+// the padded sinogram row in memory (clamped as in the unstaged kernel).
 template <typename real, bool FAN, int RY, bool STAGED>
 __device__ __forceinline__ void staged_angle(const AngleRec<real> &c, const StageWindow<real> &W,
-                                             const real *row, real xr, real yr, real *acc) {
+                                             const real *row, real dx, real dy, real *acc) {
   using A = Arith<real>;
-  const real t0 = FAN ? fma(c.c[0], xr, fma(c.c[1], yr, W.k0)) : (real)1;
-  const real q0 = FAN ? fma(c.c[3], xr, fma(c.c[4], yr, W.k1)) : fma(c.c[0], xr, fma(c.c[1], yr, W.k0));
+  const real t0 = FAN ? fma(c.c[0], dx, fma(c.c[1], dy, W.k0)) : (real)1;
+  const real q0 = FAN ? fma(W.k3, dx, fma(W.k4, dy, W.k1)) : fma(c.c[0], dx, fma(c.c[1], dy, W.k0));
 #pragma unroll
   for (int r = 0; r < RY; ++r) {
     // row r of the lane: one FMA from row 0, as in back_pixels
     const real t = FAN ? fma(c.c[1], (real)r, t0) : (real)1;
-    const real q = fma(FAN ? c.c[4] : c.c[1], (real)r, q0);
+    const real q = fma(FAN ? W.k4 : c.c[1], (real)r, q0);
     real p, w = 1;
     if (FAN) {
       w = A::rcp(t);
-      p = fma(q, w, W.k2);
+      p = STAGED ? q * w : fma(q, w, W.k2);
     } else {
       p = q;
     }
@@ -967,7 +993,8 @@ __device__ __forceinline__ void back_staged_tile(const Job<real> &J, int bx, int
   const int ya = by * TY * RY, yb = min(ya + TY * RY - 1, J.Ny + RY - 2);
   const int Nsp = J.Ns + 4;
   const real *Sz = J.S + (int64_t)z * J.Na * Nsp;
-  const real xr = (real)x, yr = (real)y0;
+  // the lane's first pixel relative to the tile corner (xa, ya)
+  const real dx = (real)(tid % TX), dy = (real)((tid / TX) * RY);
   real acc[RY];
 #pragma unroll
   for (int r = 0; r < RY; ++r) acc[r] = 0;
@@ -989,9 +1016,9 @@ __device__ __forceinline__ void back_staged_tile(const Job<real> &J, int bx, int
         const StageWindow<real> W = win[k];
         const AngleRec<real> &c = J.tab[a0 + k];
         if (W.n)
-          staged_angle<real, FAN, RY, true>(c, W, rows + k * kStageWidth, xr, yr, acc);
+          staged_angle<real, FAN, RY, true>(c, W, rows + k * kStageWidth, dx, dy, acc);
         else
-          staged_angle<real, FAN, RY, false>(c, W, Sz + (int64_t)(a0 + k) * Nsp, xr, yr, acc);
+          staged_angle<real, FAN, RY, false>(c, W, Sz + (int64_t)(a0 + k) * Nsp, dx, dy, acc);
       }
     }
     __syncthreads();

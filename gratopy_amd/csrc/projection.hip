@@ -363,7 +363,9 @@ static FwdShape fwd_shape(const Layout &L) {
   const Tuning &t = tuning();
   if (!t.automatic) {
     const bool g = t.fwd_group > 1;
-    return {t.fwd_group, g ? t.fwd_bands : 1, g ? t.fwd_chunk : 1};
+    // K = 32 exists with chunked reads only
+    const int chunk = g ? (t.fwd_group == 32 ? 4 : t.fwd_chunk) : 1;
+    return {t.fwd_group, g ? t.fwd_bands : 1, chunk};
   }
   const double w = work_of(L);
   // tiny calls: no row-padded copies (one image copy fewer to launch); the
@@ -434,6 +436,7 @@ static Job<real> make_job(bool fan, const Layout &L, const real *ofs, const real
   return J;
 }
 
+// K = 32 comes with chunked reads (16-byte strip loads) only.
 template <typename real, bool FAN, int K>
 static int launch_fwd_group(hipStream_t st, const Job<real> &J, int nb, bool chunk, real *part) {
   constexpr int CH = 16 / sizeof(real);
@@ -441,15 +444,16 @@ static int launch_fwd_group(hipStream_t st, const Job<real> &J, int nb, bool chu
   GT_REQUIRE(blocks * 256 < ((int64_t)1 << 31) && blocks * nb < ((int64_t)1 << 31),
              "too many detector groups");
   const dim3 grid((unsigned)(blocks * nb), J.Nz), block(64, 4);
-  if (nb > 1) {
-    if (chunk)
+  if (chunk) {
+    if (nb > 1)
       hipLaunchKernelGGL((fwd_band_kernel<real, FAN, K, CH>), grid, block, 0, st, J, part, nb);
     else
+      hipLaunchKernelGGL((fwd_group_kernel<real, FAN, K, CH>), grid, block, 0, st, J);
+  } else if constexpr (K <= 16) {
+    if (nb > 1)
       hipLaunchKernelGGL((fwd_band_kernel<real, FAN, K, 1>), grid, block, 0, st, J, part, nb);
-  } else if (chunk) {
-    hipLaunchKernelGGL((fwd_group_kernel<real, FAN, K, CH>), grid, block, 0, st, J);
-  } else {
-    hipLaunchKernelGGL((fwd_group_kernel<real, FAN, K, 1>), grid, block, 0, st, J);
+    else
+      hipLaunchKernelGGL((fwd_group_kernel<real, FAN, K, 1>), grid, block, 0, st, J);
   }
   return GT_OK;
 }
@@ -514,7 +518,9 @@ static int fwd_impl(hipStream_t st, real *sino, const real *img, const real *ofs
     hipLaunchKernelGGL((fwd_ray_kernel<real, FAN>), grid, dim3(64, 4), 0, st, J);
   } else {
     int rc;
-    if (K == 16)
+    if (K == 32)
+      rc = launch_fwd_group<real, FAN, 32>(st, J, nb, chunk, part);
+    else if (K == 16)
       rc = launch_fwd_group<real, FAN, 16>(st, J, nb, chunk, part);
     else if (K == 8)
       rc = launch_fwd_group<real, FAN, 8>(st, J, nb, chunk, part);

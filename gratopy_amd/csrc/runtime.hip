@@ -35,8 +35,8 @@ extern "C" int gt_tune_set(int key, int value) {
   gt::Tuning &t = gt::tuning();
   switch (key) {
     case GT_TUNE_FWD_GROUP:
-      if (value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
-        return gt::fail(GT_EINVAL, "forward group size must be 1, 2, 4, 8 or 16");
+      if (value != 1 && value != 2 && value != 4 && value != 8 && value != 16 && value != 32)
+        return gt::fail(GT_EINVAL, "forward group size must be 1, 2, 4, 8, 16 or 32");
       t.fwd_group = value;
       return GT_OK;
     case GT_TUNE_BACK_ROWS:

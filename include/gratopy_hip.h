@@ -44,7 +44,8 @@ int gt_version(void);
  * No reference counterpart: the reference leaves local sizes to the OpenCL
  * driver (gratopy/gratopy.py:304-313).  Results do not depend on them beyond
  * rounding.
- *   GT_TUNE_FWD_GROUP: detector bins per lane in the forward projection (1, 2, 4, 8, 16)
+ *   GT_TUNE_FWD_GROUP: detector bins per lane in the forward projection (1, 2, 4, 8, 16,
+ *                      32; 32 always reads 16 bytes per load)
  *   GT_TUNE_BACK_ROWS: pixels per lane in the backprojection (1, 2, 4, 8)
  *   GT_TUNE_FWD_BANDS: row bands of the forward strip walk (1, 2, 4, 8, 16, 32; applies
  *                      when GT_TUNE_FWD_GROUP > 1).  With B > 1 bands, consecutive thread
@@ -287,7 +288,8 @@ int gt_tv_adj_lambda(gt_stream_t stream, int dtype, void *lamb, void *fb, void *
  * the single-problem entry points and elementwise TV kernels under the same
  * tuning knobs.  The forward bin group, row bands and chunked strip reads,
  * the backprojection rows per lane and LDS staging (chunk, tile width) are
- * taken from the tuning knobs (gt_tune_set) when the plan is created; angle
+ * taken from the tuning knobs (gt_tune_set) when the plan is created, with at
+ * most 16 bins per lane and 8 row bands; angle
  * bands of the backprojection do not apply.
  */
 typedef struct gt_batch_item {

@@ -79,3 +79,17 @@ def rel_err(got, ref):
     """Normwise max|got - ref| / max|ref|."""
     ref = np.asarray(ref, dtype=np.float64)
     return float(np.abs(np.asarray(got, dtype=np.float64) - ref).max() / max(np.abs(ref).max(), 1e-300))
+
+
+def record_parity(config, quantity, **values):
+    """Print one measured set of parity numbers and append it as a JSON line to
+    gpurun_out/parity_records.jsonl (collected into profiles/ by hand)."""
+    import json
+    import os
+
+    line = json.dumps({"config": config, "quantity": quantity, **values}, default=float)
+    print("parity", line, flush=True)
+    out = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
+    os.makedirs(out, exist_ok=True)
+    with open(os.path.join(out, "parity_records.jsonl"), "a") as fh:
+        fh.write(line + "\n")

@@ -84,9 +84,9 @@ def test_tuning_knobs_accept_their_values_and_reject_others(lib):
     from gratopy_amd import autotune
 
     before = autotune.current_variant()
-    valid = {"fwd_group": (1, 2, 4, 8, 16), "back_rows": (1, 2, 4, 8), "fwd_bands": (1, 2, 4, 8, 16, 32),
-             "back_bands": (1, 2, 4, 8), "back_stage": (0, 8, 16), "back_tile": (16, 32),
-             "fwd_chunk": (1, 4), "automatic": (0, 1)}
+    valid = {"fwd_group": (1, 2, 4, 8, 16, 32), "back_rows": (1, 2, 4, 8),
+             "fwd_bands": (1, 2, 4, 8, 16, 32), "back_bands": (1, 2, 4, 8), "back_stage": (0, 8, 16),
+             "back_tile": (16, 32), "fwd_chunk": (1, 4), "automatic": (0, 1)}
     try:
         assert set(valid) == set(autotune.KNOBS)
         for name, values in valid.items():
@@ -105,6 +105,9 @@ def test_tuning_knobs_accept_their_values_and_reject_others(lib):
         assert padded >= 4 * ((100 + 4) * 60 + (60 + 4) * 100)
         autotune.set_variant(fwd_chunk=1)
         assert lib.gt_projection_workspace_bytes(1, 0, 0, 100, 60, 80, 30, 1, 0) < padded
+        # 32 bins per lane always read 16 bytes per load (padded copies)
+        autotune.set_variant(fwd_group=32)
+        assert lib.gt_projection_workspace_bytes(1, 0, 0, 100, 60, 80, 30, 1, 0) == padded
     finally:
         autotune.set_variant(**before)
 

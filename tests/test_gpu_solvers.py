@@ -13,7 +13,7 @@ import torch
 import gratopy_amd as gratopy
 from gratopy_amd import _device as dev
 from oracle import solvers as osolv
-from tests.helpers import assert_control, phantom, rel_err
+from tests.helpers import assert_control, phantom, record_parity, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -52,17 +52,18 @@ def _norm(geometry, g):
 
 def _vs_float64(run, sino32, label):
     """A float32 solver run judged against the float64 oracle on the same
-    (float32-valued) data: its error may be at most 3x that of the
-    reference's algorithm executed in float32 (the oracle in float32), plus
-    1e-4 of the solution's size.  Both errors are printed."""
+    (float32-valued) data (the truth): within the float32 bar 1e-5 of it, or
+    no further than 1.5x the reference's algorithm executed in float32 (the
+    oracle in float32), and within 1e-5 of that float32 run.  The three
+    distances are recorded."""
     order = "C" if sino32.flags.c_contiguous and not sino32.flags.f_contiguous else "F"
     truth = run(np.require(sino32.astype(np.float64), None, order), oracle_side=True)
     ref32 = run(sino32, oracle_side=True)
     got = run(sino32, oracle_side=False)
-    e_gpu, e_ref = rel_err(got, truth), rel_err(ref32, truth)
-    print(f"{label}: float32 error vs float64 oracle: hip {e_gpu:.3e}, reference algorithm {e_ref:.3e}",
-          flush=True)
-    assert e_gpu <= 3 * e_ref + 1e-4
+    e_gpu, e_ref, e_gr = rel_err(got, truth), rel_err(ref32, truth), rel_err(got, ref32)
+    record_parity("solvers-48x40", label, e_hip=e_gpu, e_ref=e_ref, e_hip_ref=e_gr)
+    assert e_gpu <= max(1e-5, 1.5 * e_ref), (e_gpu, e_ref)
+    assert e_gr <= 1e-5, e_gr
 
 
 @pytest.mark.parametrize("geometry", ["radon", "fan"])
@@ -80,7 +81,9 @@ def test_landweber_matches_oracle(geometry, dtype):
                                              norm_estimate=norm))
 
     if dtype == F64:
-        assert rel_err(run(sino, False), run(sino, True)) < 1e-10
+        e = rel_err(run(sino, False), run(sino, True))
+        record_parity("solvers-48x40", f"landweber {geometry} f64", e_hip_ref=e)
+        assert e < 1e-12, e
     else:
         _vs_float64(run, sino, f"landweber {geometry}")
 
@@ -101,7 +104,13 @@ def test_cg_matches_oracle(geometry, dtype, order):
         return dev.to_host(got)
 
     if dtype == F64:
-        assert rel_err(run(sino, False), run(sino, True)) < 1e-8
+        # CG amplifies rounding differences between implementations (about
+        # tenfold every three iterations on these normal equations; see
+        # test_gpu_configs.py::test_config2_cg_float64): after 8 iterations
+        # float64 runs agree to ~1e-12 only to within that growth
+        e = rel_err(run(sino, False), run(sino, True))
+        record_parity("solvers-48x40", f"cg {geometry} {order} f64", e_hip_ref=e)
+        assert e < 1e-10, e
         return
     # Float32 CG on the normal equations of this under-determined problem
     # amplifies any change of rounding: perturbing the oracle's projections by
@@ -149,7 +158,9 @@ def test_tv_matches_oracle(geometry, dtype):
                                                    number_iterations=40, norm_estimate=norm))
 
     if dtype == F64:
-        assert rel_err(run(sino, False), run(sino, True)) < 1e-9
+        e = rel_err(run(sino, False), run(sino, True))
+        record_parity("solvers-48x40", f"tv {geometry} f64", e_hip_ref=e)
+        assert e < 1e-12, e
     else:
         _vs_float64(run, sino, f"tv {geometry}")
 
@@ -165,7 +176,9 @@ def test_tv_3d_coupled_matches_oracle(order):
                                 norm_estimate=norm)
     got = gratopy.total_variation(dev.to_device(sino, "cuda:0", order=order), ps, mu=20.0,
                                   number_iterations=25, slice_thickness=0.8, norm_estimate=norm)
-    assert rel_err(dev.to_host(got), ref) < 1e-9
+    e = rel_err(dev.to_host(got), ref)
+    record_parity("solvers-48x40", f"tv 3-D coupled {order} f64", e_hip_ref=e)
+    assert e < 1e-12, e
 
 
 def test_normest_matches_oracle_and_spectrum():
