@@ -260,11 +260,27 @@ struct RowChunk {
   real v[CH];
 };
 
+// A pointer into global memory.  Loads through it stay global_load: through
+// a generic pointer they would be flat loads, which also count against the
+// LDS wait counter and so would stall every LDS access of the strip walk.
+template <typename real>
+using GPtr = const __attribute__((address_space(1))) real *;
+
 template <typename real, int CH>
-__device__ __forceinline__ RowChunk<real, CH> load_chunk(const real *p) {
+__device__ __forceinline__ RowChunk<real, CH> load_chunk(GPtr<real> p) {
   RowChunk<real, CH> c;
-  __builtin_memcpy(c.v, p, sizeof(c.v));
+  __builtin_memcpy(&c, (const __attribute__((address_space(1))) RowChunk<real, CH> *)p, sizeof(c));
   return c;
+}
+
+// px += CH for the strip walks; the empty asm keeps px itself as the loop's
+// induction variable (otherwise the compiler rebuilds the address from the
+// row start, a step counter and the first pixel's offset on every chunk, two
+// VALU instructions more)
+template <typename real, int CH>
+__device__ __forceinline__ void advance(GPtr<real> &px) {
+  px += CH;
+  asm volatile("" : "+v"(px));
 }
 
 // ---------------------------------------------------------------------------
@@ -522,38 +538,43 @@ __device__ __forceinline__ void fan_fwd_group(const Job<real> &J, int s0, int a,
   fan_rows((double)F.qx, (double)F.qy, (double)F.dpx, (double)F.dmx, (double)F.midx,
            (double)F.midy, F.Nxx, F.Nyy, ylo, yhi);
   if (BANDED) band_rows(F.Nyy, B.index, B.count, ylo, yhi);
-  const real xcap = (real)(F.Nxx - 1) - F.midx;
+  // Strip bounds in pixel columns: the bounding rays cross row y at
+  // qx + midx + dpx (y - midy - qy) and qx + midx + dmx (y - midy - qy),
+  // clamped to the image (the fmax/fmin also keep infinities, rays parallel
+  // to the rows, away from the int conversion).
+  const real qxm = F.qx + F.midx, yq = F.midy + F.qy;
+  const real xmax = (real)(F.Nxx - 1);
   // The detector position relative to bin s0 - 2 (the slot index) is
   // p = q/t with q = ss + (mid_det - s0 + 2)*t.  q is carried instead of ss
   // (|ss/t| is up to Ns/2, q/t lies in [1, K+2]), so single precision does
-  // not lose the bin fraction to cancellation; each row starts from double
-  // values: t and q at pixel column 0 of row ylo (t00, q00) plus the row and
-  // column terms, one FMA each (no running sum: q at column 0 is up to ~Ns in
-  // size, so summing row steps would drift).  p is positive, so truncation
-  // is floor.
+  // not lose the bin fraction to cancellation; each row starts from q at
+  // pixel column 0 of row ylo (q00) plus the row and column terms, in double,
+  // one FMA each (no running sum: q at column 0 is up to ~Ns in size, so
+  // summing row steps would drift).  t is of order one with terms no larger,
+  // so its row start is formed in the working precision.  p is positive, so
+  // truncation is floor.
   const double c0 = (double)J.geo[5] - (double)s0 + 2.0;
   const double tdx = F.ddx, tdy = F.ddy;
   const double qdx = (double)F.dx + c0 * tdx, qdy = (double)F.dy + c0 * tdy;
   const real dq = (real)qdx;
   const real lo = (real)0.5, hi = (real)K + (real)2.5;
   const double ud0 = (double)ylo - (double)F.midy;
-  const double t00 = tdy * ud0 + (double)F.RE - tdx * (double)F.midx;
+  const real t00 = (real)(tdy * ud0 + (double)F.RE - tdx * (double)F.midx);
   const double q00 = qdy * ud0 + c0 * (double)F.RE - qdx * (double)F.midx;
   // row y starts at F.base + y * F.row_stride, walked by one add per row
   const real *row = F.base + (int64_t)ylo * F.row_stride;
   for (int y = ylo; y <= yhi; ++y, row += F.row_stride) {
-    const real u = (real)y - F.midy;
-    const real xhigh = F.qx + F.dpx * (u - F.qy);
-    const real xlow = F.qx + F.dmx * (u - F.qy);
-    const int xh = (int)floor(fmax(fmin(xcap, xhigh) + F.midx, (real)-1));
-    const int xl = (int)ceil(fmin(fmax(-F.midx, xlow) + F.midx, (real)F.Nxx));
+    const real w = (real)y - yq;
+    const int xh = (int)floor(fmax(fmin(xmax, fma(F.dpx, w, qxm)), (real)-1));
+    const int xl = (int)ceil(fmin(fmax((real)0, fma(F.dmx, w, qxm)), (real)F.Nxx));
     if (xl > xh) continue;
-    const double dyr = (double)(y - ylo), xd = (double)xl;
-    real t = (real)fma(tdx, xd, fma(tdy, dyr, t00));
-    real q = (real)fma(qdx, xd, fma(qdy, dyr, q00));
-    // walked by address: the end pointer doubles as the loop bound
-    const real *end = row + xh;
-    for (const real *px = row + xl; px <= end; px += CH) {
+    const int dy = y - ylo;
+    real t = fma(F.ddx, (real)xl, fma(F.ddy, (real)dy, t00));
+    real q = (real)fma(qdx, (double)xl, fma(qdy, (double)dy, q00));
+    // walked by address (xl <= xh here): the end pointer is the loop bound
+    const GPtr<real> end = (GPtr<real>)(row + xh);
+    GPtr<real> px = (GPtr<real>)(row + xl);
+    do {
       const RowChunk<real, CH> c = load_chunk<real, CH>(px);
 #pragma unroll
       for (int i = 0; i < CH; ++i) {
@@ -563,7 +584,8 @@ __device__ __forceinline__ void fan_fwd_group(const Job<real> &J, int s0, int a,
         t += F.ddx;
         q += dq;
       }
-    }
+      advance<real, CH>(px);
+    } while (px <= end);
   }
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -616,19 +638,17 @@ __device__ __forceinline__ void radon_fwd_group(const Job<real> &J, int s0, int 
              Nyy, ylo, yhi);
   if (BANDED) band_rows(Nyy, B.index, B.count, ylo, yhi);
   const real lo = (real)0.5, hi = (real)K + (real)2.5;
-  const real sl = (real)(s0 - 1), sh = (real)(s0 + K);
+  // the strip's rays through s0 - 1 and s0 + K, ordered by the column they
+  // reach first along a row (inv = 1/X carries the direction)
+  const real s_first = inv < (real)0 ? (real)(s0 + K) : (real)(s0 - 1);
+  const real s_last = inv < (real)0 ? (real)(s0 - 1) : (real)(s0 + K);
   const double p00 = (double)off - (double)s0 + 2.0;  // pixel (0, 0) relative to bin s0 - 2
   // row y starts at base + y * row_stride, walked by one add per row
   const real *row = base + (int64_t)ylo * row_stride;
   for (int y = ylo; y <= yhi; ++y, row += row_stride) {
     const real d = (real)y * Y + off;  // detector position of (0, y)
-    int xl = (int)((sl - d) * inv);
-    int xh = (int)((sh - d) * inv);
-    if (inv < (real)0) {
-      const int t = xl;
-      xl = xh;
-      xh = t;
-    }
+    int xl = (int)((s_first - d) * inv);
+    int xh = (int)((s_last - d) * inv);
     xl = xl < 0 ? 0 : xl;
     xh = xh > Nxx - 1 ? Nxx - 1 : xh;
     if (xl > xh) continue;
@@ -636,15 +656,17 @@ __device__ __forceinline__ void radon_fwd_group(const Job<real> &J, int s0, int 
     // evaluated in double: x*X and y*Y + off are up to Ns in size and nearly
     // cancel; inside the strip it lies in [1, K+2], so truncation is floor
     real pr = (real)fma((double)X, (double)xl, fma((double)Y, (double)y, p00));
-    const real *end = row + xh;
-    for (const real *px = row + xl; px <= end; px += CH) {
+    const GPtr<real> end = (GPtr<real>)(row + xh);
+    GPtr<real> px = (GPtr<real>)(row + xl);
+    do {
       const RowChunk<real, CH> c = load_chunk<real, CH>(px);
 #pragma unroll
       for (int i = 0; i < CH; ++i) {
         deposit(acc, A::clamp(pr, lo, hi), c.v[i]);
         pr += X;
       }
-    }
+      advance<real, CH>(px);
+    } while (px <= end);
   }
   const real scale = radon_scale(J);
 #pragma unroll

@@ -6,13 +6,15 @@ adjoint and checkpointing.
 number of steps and returns u_N as a torch tensor that carries a gradient
 with respect to ``sino``.
 
-Forward: the iteration runs on the HIP kernels; the full state
-(u, u_bar, v, lambda) is stored every ``checkpoint`` iterations.
+Forward: the iteration runs in place on the HIP kernels; the full state
+(u, u_bar, v, lambda) is copied into a checkpoint slot every ``checkpoint``
+iterations.
 Backward: for each segment, last to first, the forward steps are recomputed
-from the segment's checkpoint (keeping every state of that segment), then the
-vector-Jacobian product of each step is applied in reverse with the adjoint
-kernels of the C ABI (gt_tv_adj_*), one forward projection (B^T) and one
-backprojection (K^T) per step.  The step map is piecewise linear: the
+from the segment's checkpoint into a pool of states reused by every segment,
+then the vector-Jacobian product of each step is applied in reverse, in
+place on one set of cotangents, with the adjoint kernels of the C ABI
+(gt_tv_adj_*), one forward projection (B^T) and one backprojection (K^T) per
+step.  Each pass allocates its buffers once, before its first step.  The step map is piecewise linear: the
 primal clamp max(0, .) contributes the mask [u_{i+1} > 0] and the dual ball
 projection its Jacobian (I - a a^T/|a|^2)/|a| where it was active; at the
 measure-zero kinks one side's derivative is taken.
@@ -54,76 +56,97 @@ def _dims(ps, sino):
     return Nx, Ny, Nz
 
 
-def _step(ps, prm, f, state, order):
-    """One primal-dual step; returns the new state (fresh tensors)."""
-    u, ubar, v, lam = state
+class _Buffers:
+    """Every device buffer of one unrolled run, allocated once per pass: the
+    working state (u, u_bar, v, lambda), checkpoint states, a segment pool
+    for the backward recomputation, the cotangent state and the projection
+    scratch.  Each step then only launches kernels."""
+
+    def __init__(self, ps, f, order):
+        self.ps, self.f, self.order = ps, f, order
+        Nx, Ny, Nz = _dims(ps, f)
+        self.img_shape = (Nx, Ny) + ((Nz,) if f.dim() > 2 else ())
+        self.dtype = dev.to_np_dtype(f.dtype)
+        self.ku = self.sino()
+        self.kstar = self.img()
+
+    def img(self):
+        return dev.zeros(self.img_shape, self.dtype, order=self.order, device=self.f.device)
+
+    def sino(self):
+        return dev.zeros(tuple(self.f.shape), self.dtype, order=self.order, device=self.f.device)
+
+    def dual(self):
+        return torch.zeros(4 * int(np.prod(self.img_shape)), dtype=self.f.dtype, device=self.f.device)
+
+    def state(self):
+        return [self.img(), self.img(), self.dual(), self.sino()]
+
+
+def _copy_state(dst, src):
+    for d, s in zip(dst, src):
+        d.copy_(s)
+
+
+def _step(ps, prm, f, src, dst, buf):
+    """One primal-dual step from state `src` = [u, u_bar, v, lambda] into
+    `dst` (dst is src: in place, with u and u_bar swapping buffers)."""
     s = ps.queue.torch_stream()
     dcode = _hip.dtype_code(f.dtype)
-    ocode = dev.order_code(order)
+    ocode = dev.order_code(buf.order)
     Nx, Ny, Nz = _dims(ps, f)
     Ns, Na = ps.sinogram_shape
-    with torch.cuda.stream(s):
-        v2 = v.clone()
-        lam2 = lam.clone(memory_format=torch.preserve_format)
-        u_new = torch.empty_like(u)
-        u_ext = u.clone(memory_format=torch.preserve_format)
-    _hip.call("gt_tv_dual", s.cuda_stream, dcode, v2.data_ptr(), ubar.data_ptr(), prm.sigma, prm.zd,
+    u, ubar, v, lam = src
+    if dst is not src:
+        with torch.cuda.stream(s):
+            dst[2].copy_(v)
+            dst[3].copy_(lam)
+            dst[1].copy_(u)
+        v, lam = dst[2], dst[3]
+    _hip.call("gt_tv_dual", s.cuda_stream, dcode, v.data_ptr(), ubar.data_ptr(), prm.sigma, prm.zd,
               Nx, Ny, Nz, ocode)
-    ku = forwardprojection(ubar, ps)
-    _hip.call("gt_tv_update_lambda", s.cuda_stream, dcode, lam2.data_ptr(), ku.data_ptr(), f.data_ptr(),
-              prm.sigma_p, prm.c, Ns, Na, Nz, ocode)
-    kstar = backprojection(lam2, ps)
-    _hip.call("gt_tv_primal", s.cuda_stream, dcode, u_new.data_ptr(), u_ext.data_ptr(), v2.data_ptr(),
-              kstar.data_ptr(), prm.tau, prm.n, prm.zd, Nx, Ny, Nz, ocode)
-    return (u_new, u_ext, v2, lam2)
+    forwardprojection(ubar, ps, sino=buf.ku)
+    _hip.call("gt_tv_update_lambda", s.cuda_stream, dcode, lam.data_ptr(), buf.ku.data_ptr(),
+              f.data_ptr(), prm.sigma_p, prm.c, Ns, Na, Nz, ocode)
+    backprojection(lam, ps, img=buf.kstar)
+    if dst is src:
+        # u_bar (consumed above) receives the new u, u the extrapolation
+        _hip.call("gt_tv_primal", s.cuda_stream, dcode, ubar.data_ptr(), u.data_ptr(), v.data_ptr(),
+                  buf.kstar.data_ptr(), prm.tau, prm.n, prm.zd, Nx, Ny, Nz, ocode)
+        src[0], src[1] = ubar, u
+    else:
+        _hip.call("gt_tv_primal", s.cuda_stream, dcode, dst[0].data_ptr(), dst[1].data_ptr(),
+                  v.data_ptr(), buf.kstar.data_ptr(), prm.tau, prm.n, prm.zd, Nx, Ny, Nz, ocode)
 
 
-def _adjoint_step(ps, prm, f, state, u_next, cot, fb, order):
-    """Cotangents of the step's inputs from those of its outputs.
-
-    state = (u, ubar, v, lam) before the step, u_next the step's new u,
-    cot = (ub', ubarb', vb', lamb'); fb accumulates the cotangent of f.
-    """
-    ub_n, ubarb_n, vb_n, lamb_n = cot
+def _adjoint_step(ps, prm, f, state, u_next, cot, fb, buf, scratch):
+    """Replace the cotangents `cot` = [ub', ubarb', vb', lamb'] of a step's
+    outputs by those of its inputs: in place, since the adjoint kernels read
+    and write each element at one index, except the dual cotangent, which
+    alternates with scratch[4] (gt_tv_adj_dual takes no aliased output).
+    state = [u, u_bar, v, lambda] before the step, u_next the step's new u;
+    fb accumulates the cotangent of f."""
+    ub, ubarb, vb, lamb = cot
     u, ubar, v, lam = state
+    wb, rwb, winv, bterm, ab = scratch
     s = ps.queue.torch_stream()
     dcode = _hip.dtype_code(f.dtype)
-    ocode = dev.order_code(order)
+    ocode = dev.order_code(buf.order)
     Nx, Ny, Nz = _dims(ps, f)
     Ns, Na = ps.sinogram_shape
-    dtype = dev.to_np_dtype(f.dtype)
-    ps.ensure_dtype(dtype)
-    w = ps.angle_weights_buf[dtype]
-    with torch.cuda.stream(s):
-        wb = torch.empty_like(u)
-        ub = torch.empty_like(u)
-        lamb = torch.empty_like(lam)
-        winv = torch.empty_like(lam)
-        ab = torch.empty_like(v)
-        ubarb = torch.empty_like(u)
-    _hip.call("gt_tv_adj_primal", s.cuda_stream, dcode, wb.data_ptr(), ub.data_ptr(), ub_n.data_ptr(),
-              ubarb_n.data_ptr(), u_next.data_ptr(), u.numel())
-    rwb = forwardprojection(wb, ps)
+    w = ps.angle_weights_buf[buf.dtype]
+    _hip.call("gt_tv_adj_primal", s.cuda_stream, dcode, wb.data_ptr(), ub.data_ptr(), ub.data_ptr(),
+              ubarb.data_ptr(), u_next.data_ptr(), u.numel())
+    forwardprojection(wb, ps, sino=rwb)
     _hip.call("gt_tv_adj_lambda", s.cuda_stream, dcode, lamb.data_ptr(), fb.data_ptr(), winv.data_ptr(),
-              lamb_n.data_ptr(), rwb.data_ptr(), w.data_ptr(), prm.tau * prm.n / prm.cT, prm.c,
+              lamb.data_ptr(), rwb.data_ptr(), w.data_ptr(), prm.tau * prm.n / prm.cT, prm.c,
               prm.sigma_p, Ns, Na, Nz, ocode)
-    bterm = backprojection(winv, ps)
-    _hip.call("gt_tv_adj_dual", s.cuda_stream, dcode, ab.data_ptr(), vb_n.data_ptr(), wb.data_ptr(),
+    backprojection(winv, ps, img=bterm)
+    _hip.call("gt_tv_adj_dual", s.cuda_stream, dcode, ab.data_ptr(), vb.data_ptr(), wb.data_ptr(),
               v.data_ptr(), ubar.data_ptr(), prm.sigma, prm.tau, prm.zd, Nx, Ny, Nz, ocode)
     _hip.call("gt_tv_adj_div", s.cuda_stream, dcode, ubarb.data_ptr(), bterm.data_ptr(),
               prm.sigma_p * prm.cT, ab.data_ptr(), prm.sigma, prm.zd, Nx, Ny, Nz, ocode)
-    return (ub, ubarb, ab, lamb)
-
-
-def _zero_state(ps, f, order):
-    Nx, Ny, Nz = _dims(ps, f)
-    shape = (Nx, Ny) + ((Nz,) if f.dim() > 2 else ())
-    dt = dev.to_np_dtype(f.dtype)
-    u = dev.zeros(shape, dt, order=order, device=f.device)
-    ubar = dev.zeros(shape, dt, order=order, device=f.device)
-    v = torch.zeros(4 * u.numel(), dtype=f.dtype, device=f.device)
-    lam = dev.zeros(tuple(f.shape), dt, order=order, device=f.device)
-    return (u, ubar, v, lam)
+    cot[2], scratch[4] = ab, vb
 
 
 class _UnrolledTV(torch.autograd.Function):
@@ -131,12 +154,17 @@ class _UnrolledTV(torch.autograd.Function):
     def forward(ctx, sino, ps, prm, n_iter, checkpoint):
         order = "C" if dev.is_c_contiguous(sino) else "F"
         f = sino.detach()
-        state = _zero_state(ps, f, order)
-        checkpoints = {}
+        ps.ensure_dtype(dev.to_np_dtype(f.dtype))
+        s = ps.queue.torch_stream()
+        with torch.cuda.stream(s):
+            buf = _Buffers(ps, f, order)
+            state = buf.state()
+            checkpoints = [buf.state() for _ in range(0, n_iter, checkpoint)]
         for i in range(n_iter):
             if i % checkpoint == 0:
-                checkpoints[i] = state
-            state = _step(ps, prm, f, state, order)
+                with torch.cuda.stream(s):
+                    _copy_state(checkpoints[i // checkpoint], state)
+            _step(ps, prm, f, state, state, buf)
         ctx.ps, ctx.prm, ctx.n_iter, ctx.checkpoint, ctx.order = ps, prm, n_iter, checkpoint, order
         ctx.checkpoints = checkpoints
         ctx.save_for_backward(f)
@@ -149,19 +177,19 @@ class _UnrolledTV(torch.autograd.Function):
         n_iter, k = ctx.n_iter, ctx.checkpoint
         s = ps.queue.torch_stream()
         with torch.cuda.stream(s):
-            fb = torch.zeros_like(f)
-            ub = grad_u.clone(memory_format=torch.preserve_format)
-            if dev.order_of(ub) != order:
-                ub = dev.to_device(dev.to_host(ub), f.device, order=order)
-            zeros = _zero_state(ps, f, order)
-            cot = (ub, zeros[1], zeros[2], zeros[3])
+            buf = _Buffers(ps, f, order)
+            fb = buf.sino()
+            cot = buf.state()
+            cot[0].copy_(grad_u)  # element by element, whatever the strides of grad_u
+            pool = [buf.state() for _ in range(min(k, n_iter))]  # states 1..k of a segment
+            scratch = [buf.img(), buf.sino(), buf.sino(), buf.img(), buf.dual()]
         for start in reversed(range(0, n_iter, k)):
             stop = min(start + k, n_iter)
-            states = [ctx.checkpoints[start]]
-            for _ in range(start, stop):
-                states.append(_step(ps, prm, f, states[-1], order))
-            for i in reversed(range(start, stop)):
-                cot = _adjoint_step(ps, prm, f, states[i - start], states[i - start + 1][0], cot, fb, order)
+            states = [ctx.checkpoints[start // k]] + pool[: stop - start]
+            for j in range(stop - start):
+                _step(ps, prm, f, states[j], states[j + 1], buf)
+            for i in reversed(range(stop - start)):
+                _adjoint_step(ps, prm, f, states[i], states[i + 1][0], cot, fb, buf, scratch)
         ctx.checkpoints = None
         return fb, None, None, None, None
 

@@ -87,6 +87,21 @@ def landweber(sino, geom, number_iterations=100, w=1.0, norm_estimate=None):
 
 def conjugate_gradients(sino, geom, number_iterations=20, epsilon=0.0, x0=None):
     """gratopy.py:2449-2551 (alpha, beta in float64)."""
+    x = None
+    for _, x in _cg_iterates(sino, geom, number_iterations, epsilon, x0):
+        pass
+    return x
+
+
+def conjugate_gradients_iterates(sino, geom, iterations):
+    """{n: x after n CG iterations} for every n in `iterations` (one run)."""
+    want = set(iterations)
+    return {n: x.copy(order="K") for n, x in _cg_iterates(sino, geom, max(want)) if n in want}
+
+
+def _cg_iterates(sino, geom, number_iterations, epsilon=0.0, x0=None):
+    """Yields (n, x) after each iteration n = 1, 2, ... of gratopy.py:2449-2551;
+    with 0 iterations, (0, x0)."""
     order = _order(sino)
     z = (sino.shape[2],) if sino.ndim > 2 else ()
     dt = sino.dtype
@@ -98,7 +113,7 @@ def conjugate_gradients(sino, geom, number_iterations=20, epsilon=0.0, x0=None):
     w = geom.angle_weights(dt)
     vdot = lambda a, b: float(np.dot(a.ravel(order="K").astype(np.float64), b.ravel(order="K").astype(np.float64)))
     sino_norm = vdot(sino, sino)
-    for _ in range(number_iterations):
+    for it in range(number_iterations):
         q = geom.fwd(p, order)
         qw = oracle.weight_sino(q, w)
         alpha = dt.type(geom.delta_x**2 / geom.delta_s * (vdot(sold, sold) / vdot(qw, q)))
@@ -108,10 +123,12 @@ def conjugate_gradients(sino, geom, number_iterations=20, epsilon=0.0, x0=None):
         beta = vdot(snew, snew) / vdot(sold, sold)
         sold, snew = snew, sold
         oracle.mul_add_add(p, beta, p.copy(order="K"), sold)
+        yield it + 1, x
         residue = np.sqrt(vdot(d, d) / sino_norm)
         if residue < epsilon:
             break
-    return x
+    if number_iterations == 0:
+        yield 0, x
 
 
 def total_variation(sino, geom, mu, number_iterations=1000, slice_thickness=1.0,
