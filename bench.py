@@ -24,6 +24,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # bytes/s, MI355X
+# VALU issue peak of MI355X in lane-operations per second: the published FP32
+# vector peak, 157.3 TFLOP/s, counts two flops per FMA lane-operation
+# (256 CUs x 128 lanes x 2.4 GHz)
+VALU_PEAK = 157.3e12 / 2
 
 WORKLOADS = {
     # name: (Nz, N, Na, Ns, RE, R, image_width, detector_width, ellipses); RE and R are
@@ -52,8 +56,13 @@ def parse():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-angles", type=int, default=90,
-                   help="angles of the bounded CPU-baseline sample (one slice)")
+    p.add_argument("--cpu-angles", type=int, default=None,
+                   help="angles of the CPU-baseline sample (one slice; default: all)")
+    p.add_argument("--unrolled", action="store_true",
+                   help="only time the unrolled TV(100) forward and backward passes, 16 x 512^2 "
+                        "float64, Radon and fanbeam")
+    p.add_argument("--cpu-configs", action="store_true",
+                   help="only time configurations 1 and 2 in full on the CPU oracle and the GPU")
     p.add_argument("--fwd-group", type=int, default=None, help="detector bins per lane (forward)")
     p.add_argument("--back-rows", type=int, default=None, help="pixels per lane (backprojection)")
     p.add_argument("--fwd-bands", type=int, default=None, help="row bands of the forward (1 or 8)")
@@ -64,15 +73,19 @@ def parse():
                    help="tile width (16 or 32 pixels) of the staged backprojection")
     p.add_argument("--fwd-chunk", type=int, default=None,
                    help="forward strip reads: 1 pixel or 4 (16 bytes) per load")
-    p.add_argument("--auto", action="store_true",
-                   help="no autotuning: the library's automatic launch shapes (its default)")
+    p.add_argument("--tune", action="store_true",
+                   help="time every launch variant on the workload first and keep the fastest "
+                        "(default: the library's automatic launch shapes, which on the default "
+                        "workload are the autotuner's choice)")
     p.add_argument("--no-batched", action="store_true",
                    help="skip the batched-vs-plain solver measurements")
     p.add_argument("--uneven-only", action="store_true",
                    help="only the batched-vs-plain Landweber and TV measurements on the 64 uneven "
                         "slices (with launch-variant flags: those variants, no batch autotuning)")
-    p.add_argument("--tv-iterations", type=int, default=6,
-                   help="TV iterations timed per path on the 2048^2 stack (rate extrapolated to 500)")
+    p.add_argument("--tv-iterations", type=int, default=500,
+                   help="TV iterations of the batched + graph and the plain per-slice paths on the stack")
+    p.add_argument("--tv-ref-iterations", type=int, default=100,
+                   help="TV iterations of the reference-structured per-slice loop on the stack")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only run an in-place scale of 1 GiB (reads and writes 1 GiB) three times, "
                         "to calibrate the FETCH_SIZE / WRITE_SIZE counters")
@@ -112,11 +125,60 @@ def _timed(fn):
     return time.perf_counter() - t0, out
 
 
-def bench_tv_stack(ps, img, n_iter, nominal=500):
-    """TV (mu=5e-3, 1% noise, slice_thickness 0) on the whole stack: batched
-    (all slices per launch) and graph-captured vs the plain per-slice loop
-    (one slice per call, eager launches).  Both paths run n_iter iterations;
-    slices/s is extrapolated to `nominal` iterations per slice."""
+def tv_reference_structured(f, ps, mu, n_iter, norm_estimate):
+    """The TV loop in the reference's structure (gratopy/gratopy.py:2744-2759):
+    seven launches per iteration through the one-to-one entry points
+    (update_v, update_NormV, forward projection, update_lambda,
+    backprojection, update_u, update_extra), scalars from the host, no
+    fusion, no graph.  2-D slice, slice_thickness 0.  Returns u."""
+    import torch
+
+    import gratopy_amd as gratopy
+    from gratopy_amd import _device as dev
+    from gratopy_amd import _hip
+
+    lib = _hip.load_library()
+    dtype = dev.to_np_dtype(f.dtype)
+    Nx, Ny = ps.img_shape
+    Ns, Na = ps.sinogram_shape
+    dc, oc = _hip.dtype_code(dtype), dev.order_code(dev.order_of(f))
+    U = dev.zeros(ps.img_shape, dtype, order="F", device=f.device)
+    U_ = dev.zeros(ps.img_shape, dtype, order="F", device=f.device)
+    V = torch.zeros(4 * Nx * Ny, dtype=f.dtype, device=f.device)
+    normV = dev.zeros(ps.img_shape, dtype, order="F", device=f.device)
+    Lamb = dev.zeros(ps.sinogram_shape, dtype, order="F", device=f.device)
+    KU = dev.zeros(ps.sinogram_shape, dtype, order="F", device=f.device)
+    KSTAR = dev.zeros(ps.img_shape, dtype, order="F", device=f.device)
+    sigma = 1.0 / np.sqrt(13) * 10.0
+    tau = 1.0 / np.sqrt(13) / 10.0
+    mu = mu * norm_estimate**2 * ps.delta_x
+    mu = mu / (sigma + mu)
+    st = torch.cuda.current_stream().cuda_stream
+    for _ in range(n_iter):
+        _hip.check(lib.gt_tv_update_v(st, dc, V.data_ptr(), U_.data_ptr(), sigma, 0.0, Nx, Ny, 1, oc),
+                   "gt_tv_update_v")
+        _hip.check(lib.gt_tv_update_normv(st, dc, V.data_ptr(), normV.data_ptr(), Nx, Ny, 1, oc),
+                   "gt_tv_update_normv")
+        gratopy.forwardprojection(U_, ps, sino=KU)
+        _hip.check(lib.gt_tv_update_lambda(st, dc, Lamb.data_ptr(), KU.data_ptr(), f.data_ptr(),
+                                           sigma / norm_estimate, mu, Ns, Na, 1, oc),
+                   "gt_tv_update_lambda")
+        gratopy.backprojection(Lamb, ps, img=KSTAR)
+        _hip.check(lib.gt_tv_update_u(st, dc, U_.data_ptr(), U.data_ptr(), V.data_ptr(), KSTAR.data_ptr(),
+                                      tau, 1.0 / norm_estimate, 0.0, Nx, Ny, 1, oc), "gt_tv_update_u")
+        _hip.check(lib.gt_tv_update_extra(st, dc, U_.data_ptr(), U.data_ptr(), Nx * Ny),
+                   "gt_tv_update_extra")
+        U, U_ = U_, U
+    return U
+
+
+def bench_tv_stack(ps, img, n_iter, n_ref):
+    """TV (mu=5e-3, 1% noise, slice_thickness 0) on the whole stack, three
+    ways: batched (all slices per launch) and graph-captured, and the plain
+    per-slice loop of total_variation (fused dual and primal updates, eager
+    launches), both for n_iter iterations; the reference-structured per-slice
+    loop (seven launches per iteration, host scalars, no graph) for n_ref
+    iterations (its slices/s is quoted per n_iter iterations)."""
     import torch
 
     import gratopy_amd as gratopy
@@ -125,27 +187,42 @@ def bench_tv_stack(ps, img, n_iter, nominal=500):
     Nz = img.shape[2]
     sino = gratopy.forwardprojection(img, ps)
     host = add_noise(dev.to_host(sino), 1.0, np.random.default_rng(7))
+    del sino
     f = dev.to_device(np.asfortranarray(host), img.device)
+    del host
     norm = gratopy.normest(ps)
     mu = 5e-3
     slices = [f[:, :, z] for z in range(Nz)]  # Fortran-ordered 2-D views
     run_batched = lambda: gratopy.total_variation(f, ps, mu, n_iter, slice_thickness=0.0,
                                                   norm_estimate=norm, graph=True)
     run_plain = lambda: [gratopy.total_variation(s, ps, mu, n_iter, norm_estimate=norm) for s in slices]
-    run_batched()  # capture / warm-up
+    run_ref = lambda: [tv_reference_structured(s, ps, mu, n_ref, norm) for s in slices]
+    gratopy.total_variation(f, ps, mu, 2, slice_thickness=0.0, norm_estimate=norm, graph=True)  # warm-up
     tb, ub = _timed(run_batched)
-    run_plain()
+    print(f"tv stack: batched + graph {tb:.1f} s", file=sys.stderr, flush=True)
     tp, up = _timed(run_plain)
-    diff = max(float((ub[:, :, z] - up[z]).abs().max()) for z in range(Nz))
-    rate_b = Nz / (tb / n_iter * nominal)
-    rate_p = Nz / (tp / n_iter * nominal)
+    print(f"tv stack: plain loop {tp:.1f} s", file=sys.stderr, flush=True)
+    same_plain = all(torch.equal(ub[:, :, z], up[z]) for z in range(Nz))
+    del up
+    tr, ur = _timed(run_ref)
+    print(f"tv stack: reference-structured loop {tr:.1f} s", file=sys.stderr, flush=True)
+    del ub
+    # the reference-structured loop against the batched path at the same iteration count
+    ub = gratopy.total_variation(f, ps, mu, n_ref, slice_thickness=0.0, norm_estimate=norm, graph=True)
+    diff_ref = max(float((ub[:, :, z] - ur[z]).abs().max()) for z in range(Nz))
     Nx, Ny, Nz = (int(n) for n in img.shape)
     stack = f"{Nz} x {Nx}^2 {'fanbeam' if ps.is_fan else 'radon'} stack"
-    return {"metric": f"TV(500) slices/s, {stack}", "batched_graph": round(rate_b, 4),
-            "plain_per_slice": round(rate_p, 4), "speedup": round(rate_b / rate_p, 3),
-            "iterations_timed": n_iter, "ms_per_iteration_batched": round(tb / n_iter * 1e3, 2),
+    return {"metric": f"TV({n_iter}) slices/s, {stack}", "batched_graph": round(Nz / tb, 4),
+            "plain_per_slice": round(Nz / tp, 4),
+            "reference_structured": round(Nz / (tr / n_ref * n_iter), 4),
+            "speedup_vs_plain": round(tp / tb, 3),
+            "speedup_vs_reference_structured": round((tr / n_ref) / (tb / n_iter), 3),
+            "iterations": n_iter, "reference_structured_iterations_timed": n_ref,
+            "ms_per_iteration_batched": round(tb / n_iter * 1e3, 2),
             "ms_per_iteration_plain": round(tp / n_iter * 1e3, 2),
-            "max_abs_diff_batched_vs_plain": diff}
+            "ms_per_iteration_reference_structured": round(tr / n_ref * 1e3, 2),
+            "bitwise_batched_vs_plain": same_plain,
+            f"max_abs_diff_batched_vs_reference_structured_at_{n_ref}": diff_ref}
 
 
 def uneven_mix(n=64, seed=3):
@@ -224,8 +301,10 @@ def bench_uneven_landweber(problem, n_iter=100):
 
 def bench_uneven_tv(problem, n_iter=100, mu=1e-2):
     """TV(100) (mu = 1e-2) on the 64-slice uneven mix: one batched plan, graph
-    captured, vs the plain per-slice loop of total_variation calls (the
-    reference's structure: seven launches per iteration with host scalars)."""
+    captured, vs two per-slice loops: total_variation (fused dual and primal
+    updates, five launches per iteration, eager) and the reference-structured
+    loop (seven launches per iteration through the one-to-one entry points,
+    host scalars, no graph)."""
     import torch
 
     import gratopy_amd as gratopy
@@ -235,30 +314,75 @@ def bench_uneven_tv(problem, n_iter=100, mu=1e-2):
     run_b = lambda: batch_total_variation(sinos, settings, mu, n_iter, norm_estimates=norms, graph=True)
     run_p = lambda: [gratopy.total_variation(f, ps, mu, n_iter, norm_estimate=n)
                      for f, ps, n in zip(sinos, settings, norms)]
+    run_r = lambda: [tv_reference_structured(f, ps, mu, n_iter, n) for f, ps, n in zip(sinos, settings, norms)]
     run_b()
     tb, ub = _timed(run_b)
     run_p()
     tp, up = _timed(run_p)
     same = all(torch.equal(a, b) for a, b in zip(ub, up))
+    run_r()
+    tr, ur = _timed(run_r)
+    diff_r = max(float((a - b).abs().max() / max(float(b.abs().max()), 1e-30)) for a, b in zip(ub, ur))
     n = len(settings)
     return {"metric": "TV(100) slices/s, 64 uneven slices (config of 128..1024 px, 60..720 angles)",
             "batched_graph": round(n / tb, 3), "plain_per_slice": round(n / tp, 3),
-            "speedup": round(tp / tb, 3), "fwd+back GUPS batched": round(work * n_iter / tb / 1e9, 2),
-            "bitwise_equal": same}
+            "reference_structured": round(n / tr, 3),
+            "speedup": round(tp / tb, 3), "speedup_vs_reference_structured": round(tr / tb, 3),
+            "fwd+back GUPS batched": round(work * n_iter / tb / 1e9, 2),
+            "bitwise_equal": same, "max_rel_diff_vs_reference_structured": diff_r}
 
 
-def measured_traffic(workload, kernel):
-    """HBM bytes per launch of `kernel` on `workload` from the committed
-    rocprofv3 --pmc summary (profiles/pmc_<kernel>_<workload>.txt, line
-    'hbm_bytes_per_launch: N'), or None when there is none."""
-    path = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{workload}.txt")
+def profile_value(name, key):
+    """The number on the line 'key: N' of the committed rocprofv3 summary
+    profiles/<name>, or None when the file or the line is absent."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         for line in fh:
-            if line.startswith("hbm_bytes_per_launch:"):
+            if line.startswith(key + ":"):
                 return float(line.split(":", 1)[1])
     return None
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of the `kernel` direction on `workload` from the
+    committed rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE summary, or None."""
+    return profile_value(f"pmc_{kernel}_{workload}.txt", "hbm_bytes_per_launch")
+
+
+def roofline(workload, direction, seconds, work, elem=4):
+    """Roofline of one direction of the projection (all its kernels, `seconds`
+    per launch, measured live).  The bound is VALU issue: lane-operations
+    (SQ_INSTS_VALU x 64, from the committed rocprofv3 --pmc summary) per
+    second against the FP32 vector peak.  Also reported: HBM traffic from the
+    FETCH_SIZE / WRITE_SIZE counters against 8 TB/s, and the algorithmic
+    figure of two interpolation taps per pixel-angle, whose bytes are mostly
+    served from LDS and the caches, not HBM."""
+    valu = profile_value(f"pmc_sq_{workload}.txt", f"valu_wave_insts_per_launch {direction}")
+    traffic = measured_traffic(workload, direction)
+    algo = 2.0 * elem * work
+    out = {}
+    if valu is not None:
+        lane_ops = valu * 64 / seconds
+        out = {"bound": "valu", "kernel": direction, "achieved": round(lane_ops / 1e12, 3),
+               "peak": round(VALU_PEAK / 1e12, 3), "unit": "T lane-op/s",
+               "frac": round(lane_ops / VALU_PEAK, 4), "traffic": traffic,
+               "valu_lane_ops_per_pixel_angle": round(valu * 64 / work, 2)}
+    elif traffic is not None:
+        out = {"bound": "hbm", "kernel": direction, "achieved": round(traffic / seconds / 1e9, 2),
+               "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(traffic / seconds / HBM_PEAK, 4),
+               "traffic": traffic}
+    else:
+        out = {"bound": None, "kernel": direction, "traffic": None,
+               "note": f"no committed counter summary for {workload}"}
+    if traffic is not None:
+        out["hbm"] = {"achieved": round(traffic / seconds / 1e9, 2), "peak": HBM_PEAK / 1e9,
+                      "unit": "GB/s", "frac": round(traffic / seconds / HBM_PEAK, 4)}
+    out["algorithmic"] = {"bytes_per_launch": algo, "achieved": round(algo / seconds / 1e9, 1),
+                          "unit": "GB/s", "of_hbm_peak": round(algo / seconds / HBM_PEAK, 3),
+                          "note": "2 taps x element size per pixel-angle; taps come from LDS/L1/L2"}
+    return out
 
 
 def make_stack(Nz, N, n_ellipses, dtype=np.float32):
@@ -271,8 +395,11 @@ def make_stack(Nz, N, n_ellipses, dtype=np.float32):
 
 
 def cpu_baseline(fan, ps_kwargs, N, n_angles, n_det, slice_img):
-    """Oracle (C restatement of the reference kernels, OpenMP) on one slice
-    with a subset of the angles; returns pixel-angle updates per second."""
+    """The oracle's timing build (the reference kernels restated in C,
+    OpenMP over output elements, FMA contraction allowed, AVX-512 where the
+    host has it) on one slice of the stack with all its angles; returns
+    pixel-angle updates per second (the rate of the whole stack, whose slices
+    are independent) and the two directions' times."""
     import gratopy_amd as gratopy
     import oracle
 
@@ -281,21 +408,119 @@ def cpu_baseline(fan, ps_kwargs, N, n_angles, n_det, slice_img):
     t = ps.struct
     dt = np.dtype("float32")
     img = np.asfortranarray(slice_img)
-    oracle.lib()
-    t0 = time.perf_counter()
-    if fan:
-        sino = oracle.fan_fwd(img, t["ofs_dict"][dt], t["sdpd_dict"][dt], t["geo_dict"][dt], n_det,
-                              n_angles)
-    else:
-        sino = oracle.radon_fwd(img, t["ofs_dict"][dt], t["geo_dict"][dt], n_det, n_angles)
-    t1 = time.perf_counter()
-    if fan:
-        oracle.fan_back(sino, t["ofs_dict"][dt], t["sdpd_dict"][dt], t["geo_dict"][dt], (N, N))
-    else:
-        oracle.radon_back(sino, t["ofs_dict"][dt], t["geo_dict"][dt], (N, N))
-    t2 = time.perf_counter()
+    with oracle.timing_build() as build:
+        t0 = time.perf_counter()
+        if fan:
+            sino = oracle.fan_fwd(img, t["ofs_dict"][dt], t["sdpd_dict"][dt], t["geo_dict"][dt], n_det,
+                                  n_angles)
+        else:
+            sino = oracle.radon_fwd(img, t["ofs_dict"][dt], t["geo_dict"][dt], n_det, n_angles)
+        t1 = time.perf_counter()
+        if fan:
+            oracle.fan_back(sino, t["ofs_dict"][dt], t["sdpd_dict"][dt], t["geo_dict"][dt], (N, N))
+        else:
+            oracle.radon_back(sino, t["ofs_dict"][dt], t["geo_dict"][dt], (N, N))
+        t2 = time.perf_counter()
     work = N * N * n_angles
-    return work / (t2 - t0), (t1 - t0), (t2 - t1)
+    return work / (t2 - t0), (t1 - t0), (t2 - t1), build.variant
+
+
+def cpu_threads():
+    """OpenMP threads the oracle runs with."""
+    n = os.environ.get("OMP_NUM_THREADS")
+    return int(n) if n and n.isdigit() else (os.cpu_count() or 1)
+
+
+def cpu_configs():
+    """Configurations 1 and 2 in full, timed on the CPU (the oracle's timing
+    build under the solver loops of oracle/solvers.py) and on the GPU, same
+    inputs and injected norm estimate; one JSON object."""
+    import torch
+
+    import gratopy_amd as gratopy
+    import oracle
+    from gratopy_amd import _device as dev
+    from gratopy_amd.phantom import random_ellipses
+    from oracle import solvers as osolv
+
+    out = {"cores": cpu_threads()}
+
+    def both(label, cpu_fn, gpu_fn):
+        with oracle.timing_build() as build:
+            t0 = time.perf_counter()
+            cpu_fn()
+            tc = time.perf_counter() - t0
+        gpu_fn()  # warm-up
+        tg, _ = _timed(gpu_fn)
+        out[label] = {"cpu_s": round(tc, 3), "gpu_s": round(tg, 5), "gpu_over_cpu": round(tc / tg, 1),
+                      "cpu_build": build.variant}
+        print(f"{label}: cpu {tc:.2f} s, gpu {tg * 1e3:.2f} ms", file=sys.stderr, flush=True)
+
+    # configuration 1: Radon 256^2, 180 angles, 256 detectors, float32
+    ps = gratopy.ProjectionSettings("cuda:0", gratopy.RADON, (256, 256), 180, 256)
+    g = osolv.OracleGeometry.from_settings(ps)
+    img = np.asfortranarray(random_ellipses(256, 10, seed=101, dtype=np.float32))
+    sino = g.fwd(img)
+    img_d, sino_d = dev.to_device(img, "cuda:0"), dev.to_device(sino, "cuda:0")
+    np.random.seed(0)
+    norm = float(gratopy.normest(ps))
+    both("config1_forward", lambda: g.fwd(img), lambda: gratopy.forwardprojection(img_d, ps, sino=sino_d.clone()))
+    both("config1_backprojection", lambda: g.back(sino), lambda: gratopy.backprojection(sino_d, ps))
+    both("config1_landweber100", lambda: osolv.landweber(sino, g, 100, norm_estimate=norm),
+         lambda: gratopy.landweber(sino_d, ps, 100, norm_estimate=norm))
+    # configuration 2: fanbeam 512^2, 360 angles, 768 detectors, 1% noise
+    ps = gratopy.ProjectionSettings("cuda:0", gratopy.FANBEAM, (512, 512), 360, 768, RE=400.0, R=800.0,
+                                    image_width=512.0, detector_width=1024.0)
+    g = osolv.OracleGeometry.from_settings(ps)
+    clean = g.fwd(np.asfortranarray(random_ellipses(512, 15, seed=102, dtype=np.float64)))
+    data32 = np.asfortranarray(add_noise(clean, 1.0, np.random.default_rng(202)).astype(np.float32))
+    np.random.seed(0)
+    norm = float(gratopy.normest(ps, dtype="float64"))
+    for dtype in (np.float32, np.float64):
+        data = np.asfortranarray(data32.astype(dtype))
+        d = dev.to_device(data, "cuda:0")
+        name = np.dtype(dtype).name
+        both(f"config2_cg50_{name}", lambda: osolv.conjugate_gradients(data, g, 50),
+             lambda: gratopy.conjugate_gradients(d, ps, 50))
+        both(f"config2_tv200_{name}", lambda: osolv.total_variation(data, g, 1e-2, 200, norm_estimate=norm),
+             lambda: gratopy.total_variation(d, ps, 1e-2, 200, norm_estimate=norm))
+    torch.cuda.synchronize()
+    return out
+
+
+def bench_unrolled(geometry="fan", nz=16, n_iter=100, checkpoint=10):
+    """The differentiable unrolled TV at the differentiation configuration's
+    size: 512^2, 360 angles, 768 detectors, float64, a batch of `nz` slices,
+    `n_iter` iterations with checkpoints every `checkpoint`: seconds for the
+    forward pass and for the backward pass (recomputation + adjoint)."""
+    import torch
+
+    import gratopy_amd as gratopy
+    from gratopy_amd import _device as dev
+    from gratopy_amd.phantom import random_ellipses
+    from gratopy_amd.tv_unrolled import tv_unrolled
+
+    if geometry == "fan":
+        ps = gratopy.ProjectionSettings("cuda:0", gratopy.FANBEAM, (512, 512), 360, 768, R=800.0,
+                                        RE=400.0, detector_width=1024.0, image_width=512.0)
+    else:
+        ps = gratopy.ProjectionSettings("cuda:0", gratopy.RADON, (512, 512), 360, 768)
+    imgs = np.stack([random_ellipses(512, 15, seed=900 + z, dtype=np.float64) for z in range(nz)], -1)
+    f = gratopy.forwardprojection(dev.to_device(np.asfortranarray(imgs), "cuda:0"), ps)
+    np.random.seed(0)
+    norm = float(gratopy.normest(ps, dtype="float64"))
+    run = lambda x: tv_unrolled(x, ps, 1e-2, n_iter, norm_estimate=norm, checkpoint=checkpoint)
+    x = f.clone().requires_grad_(True)
+    run(x).sum().backward()  # warm-up
+    x = f.clone().requires_grad_(True)
+    tf, u = _timed(lambda: run(x))
+    tb, _ = _timed(lambda: u.sum().backward())
+    W = 512 * 512 * 360 * nz
+    return {"metric": f"unrolled TV({n_iter}) forward + backward, {nz} x 512^2 {geometry}, float64",
+            "forward_s": round(tf, 3), "backward_s": round(tb, 3),
+            "backward_over_forward": round(tb / tf, 2), "checkpoint": checkpoint,
+            "projections_per_pass": {"forward": 2 * n_iter, "backward": 4 * n_iter},
+            "fwd+back GUPS forward pass": round(2 * n_iter * W / 2 / tf / 1e9, 1)}
 
 
 def main():
@@ -311,6 +536,13 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.cpu_configs:
+        print(json.dumps({"cpu_configs": cpu_configs()}), file=json_out, flush=True)
+        return
+    if args.unrolled:
+        print(json.dumps({"unrolled": [bench_unrolled("fan"), bench_unrolled("radon")]}), file=json_out,
+              flush=True)
         return
 
     import torch
@@ -348,7 +580,7 @@ def main():
     stream = torch.cuda.current_stream()
     tuning_source = "fixed"
     tuned = None
-    if args.auto and not fixed:
+    if not args.tune and not fixed:
         tuning_source = "automatic"
     elif not fixed:
         back.copy_(img)
@@ -385,17 +617,16 @@ def main():
     # 2 interpolation taps x 4 B per pixel-angle per direction
     fwd_med, back_med = float(np.median(t_fwd)), float(np.median(t_back))
     dom_name, dom_t = ("forward", fwd_med) if fwd_med >= back_med else ("backprojection", back_med)
-    algo_bytes = 8.0 * work
-    achieved = algo_bytes / dom_t
 
     cpu = None
     if not args.no_cpu_baseline:
-        n_ang = min(args.cpu_angles, Na)
-        rate, tf, tb = cpu_baseline(fan, ps_kwargs, N, n_ang, Ns, host[:, :, 0])
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        cpu = {"value": rate / 1e9, "unit": "GUPS", "cores": threads, "kind": "port",
-               "sample": f"oracle {'fan' if fan else 'radon'} fwd+back, 1 slice {N}x{N}, "
-                         f"{n_ang} of {Na} angles, {Ns} det (fwd {tf:.2f}s, back {tb:.2f}s)"}
+        n_ang = Na if args.cpu_angles is None else min(args.cpu_angles, Na)
+        rate, tf, tb, variant = cpu_baseline(fan, ps_kwargs, N, n_ang, Ns, host[:, :, 0])
+        cpu = {"value": round(rate / 1e9, 4), "unit": "GUPS", "cores": cpu_threads(), "kind": "port",
+               "sample": f"oracle timing build ({variant}, OpenMP, FMA) {'fan' if fan else 'radon'} "
+                         f"fwd+back of 1 of the {Nz} slices, {N}x{N}, {n_ang} of {Na} angles, {Ns} det "
+                         f"(fwd {tf:.2f}s, back {tb:.2f}s); slices are independent, so the stack's "
+                         f"rate is the slice's"}
 
     out = {
         "metric": ("fwd+back GUPS (pixel*angle/s) fanbeam 2048^2" if args.workload == DEFAULT_WORKLOAD
@@ -423,15 +654,14 @@ def main():
         "fwd_ms": round(fwd_med * 1e3, 3),
         "back_ms": round(back_med * 1e3, 3),
         "wall_s": round(wall, 3),
-        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved / 1e9,
-                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": measured_traffic(args.workload, dom_name)},
+        "roofline": roofline(args.workload, dom_name, dom_t, work),
+        "gups_target_50pct_of_hbm_model": 250.0,
         "cpu_baseline": cpu,
     }
     if not args.no_batched:
         del sino, back
         torch.cuda.empty_cache()
-        out["batched"] = {"tv_stack": bench_tv_stack(ps, img, args.tv_iterations)}
+        out["batched"] = {"tv_stack": bench_tv_stack(ps, img, args.tv_iterations, args.tv_ref_iterations)}
         print("tv stack measured", file=sys.stderr, flush=True)
         del img
         torch.cuda.empty_cache()

@@ -816,8 +816,8 @@ __device__ __forceinline__ void back_pixels(const Job<real> &J, int x, int y0, i
     const real q0 = FAN ? fma(c.c[3], xr, fma(c.c[4], yr, c.c[5])) : fma(c.c[0], xr, fma(c.c[1], yr, c.c[2]));
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
-      const real t = FAN ? fma(c.c[1], (real)r, t0) : (real)1;
-      const real q = fma(FAN ? c.c[4] : c.c[1], (real)r, q0);
+      const real t = FAN ? (r == 0 ? t0 : fma(c.c[1], (real)r, t0)) : (real)1;
+      const real q = r == 0 ? q0 : fma(FAN ? c.c[4] : c.c[1], (real)r, q0);
       real p, w = 1;
       if (FAN) {
         w = A::rcp(t);
@@ -959,9 +959,10 @@ __device__ __forceinline__ void staged_angle(const AngleRec<real> &c, const Stag
   const real q0 = FAN ? fma(W.k3, dx, fma(W.k4, dy, W.k1)) : fma(c.c[0], dx, fma(c.c[1], dy, W.k0));
 #pragma unroll
   for (int r = 0; r < RY; ++r) {
-    // row r of the lane: one FMA from row 0, as in back_pixels
-    const real t = FAN ? fma(c.c[1], (real)r, t0) : (real)1;
-    const real q = fma(FAN ? W.k4 : c.c[1], (real)r, q0);
+    // row r of the lane: one FMA from row 0, as in back_pixels (row 0 is
+    // t0, q0 themselves: fma(c, 0, t0) is not folded, IEEE zeros and NaNs)
+    const real t = FAN ? (r == 0 ? t0 : fma(c.c[1], (real)r, t0)) : (real)1;
+    const real q = r == 0 ? q0 : fma(FAN ? W.k4 : c.c[1], (real)r, q0);
     real p, w = 1;
     if (FAN) {
       w = A::rcp(t);

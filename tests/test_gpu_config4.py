@@ -8,10 +8,12 @@ R 800, image width 512 for fanbeam).
   eps = 1e-4 (TV's mu = 1e-2 for fanbeam, the same data weight for Radon):
   the gradient with respect to the measured sinograms f from the
   hand-written adjoint (gt_tv_adj_* kernels) against finite differences
-  (h = 1e-6, five-point stencil) in 32 random directions.  The iteration is
-  piecewise linear (max(0, .), unit-ball projections): a +-h step that
-  crosses a kink in some pixel changes the difference quotient slightly, so
-  the median relative error is held to 1e-6 and the largest to 1e-4.
+  (five-point stencil, step 2e-9 max|f|) in 32 random directions, on phantoms with a
+  constant background of 0.25 so that the reconstruction stays off the clamp
+  max(0, .).  The iteration is piecewise linear (that clamp, unit-ball
+  projections): a +-h step that crosses a kink in some pixel changes the
+  difference quotient slightly, so the median relative error is held to 1e-6
+  and the largest to 1e-4.
 * One slice against the gradient assembled from the oracle's operators:
   the same iteration in torch on the CPU, with R and its backprojection
   from the oracle (the reference kernels restated in C) as autograd
@@ -62,11 +64,13 @@ def _norm(ps):
     return float(gratopy.normest(ps, number_iterations=50, dtype="float64"))
 
 
-def _batch(ps, nz, seed):
-    """Phantoms, their clean sinograms y (the loss's target) and noisy
-    sinograms f (the solver's input), (Ns, Na, nz) float64 F order."""
+def _batch(ps, nz, seed, background=0.0):
+    """Phantoms (plus a constant background), their clean sinograms y (the
+    loss's target) and noisy sinograms f (the solver's input), (Ns, Na, nz)
+    float64 F order."""
     rng = np.random.default_rng(seed)
-    imgs = np.stack([random_ellipses(512, 15, seed=seed * 100 + z, dtype=F64) for z in range(nz)], axis=-1)
+    imgs = np.stack([random_ellipses(512, 15, seed=seed * 100 + z, dtype=F64) + background
+                     for z in range(nz)], axis=-1)
     y = gratopy.forwardprojection(dev.to_device(np.asfortranarray(imgs), "cuda:0"), ps)
     f = dev.to_device(np.asfortranarray(add_noise(dev.to_host(y), 1.0, rng)), "cuda:0")
     return y, f
@@ -79,7 +83,10 @@ def test_config4_unrolled_tv_finite_differences_batch16(geometry):
     ps = _ps(geometry)
     norm = _norm(ps)
     mu = _mu(ps, norm)
-    y, f0 = _batch(ps, 16, 41 if geometry == "fan" else 42)
+    # a positive background keeps the reconstruction off the clamp max(0, .):
+    # with a zero background about half of its pixels sit exactly at that
+    # kink, and +-h steps then straddle it in many pixels at once
+    y, f0 = _batch(ps, 16, 41 if geometry == "fan" else 42, background=0.25)
 
     def loss(f):
         x = tv_unrolled(f, ps, mu, ITERS, norm_estimate=norm, checkpoint=CHECKPOINT)
@@ -89,7 +96,11 @@ def test_config4_unrolled_tv_finite_differences_batch16(geometry):
     f = f0.clone().requires_grad_(True)
     loss(f).backward()
     grad = f.grad.detach()
-    h = 1e-6
+    # the step is relative to the data: 1e-6 on the fanbeam sinograms (values
+    # up to ~500 in pixel units); the parallel-beam ones are ~250x smaller
+    # (pixels 2/512 wide), where an absolute 1e-6 would cross ~250x more
+    # kinks of the iteration
+    h = 2e-9 * float(f0.abs().max())
     errs = []
     with torch.no_grad():
         for k in range(32):
@@ -102,7 +113,7 @@ def test_config4_unrolled_tv_finite_differences_batch16(geometry):
             an = float((grad * d).sum())
             errs.append(abs(fd - an) / abs(an))
     errs = np.array(errs)
-    record_parity("config4", f"unrolled_tv100_fd_{geometry}", batch=16, directions=32, mu=mu,
+    record_parity("config4", f"unrolled_tv100_fd_{geometry}", batch=16, directions=32, mu=mu, h=h,
                   median_rel_err=float(np.median(errs)), max_rel_err=float(errs.max()))
     assert np.median(errs) < 1e-6 and errs.max() < 1e-4, errs
 
