@@ -73,6 +73,8 @@ def parse():
                    help="tile width (16 or 32 pixels) of the staged backprojection")
     p.add_argument("--fwd-chunk", type=int, default=None,
                    help="forward strip reads: 1 pixel or 4 (16 bytes) per load")
+    p.add_argument("--fwd-compact", type=int, default=None,
+                   help="1: banded forward launches only the bin groups that meet each band")
     p.add_argument("--tune", action="store_true",
                    help="time every launch variant on the workload first and keep the fastest "
                         "(default: the library's automatic launch shapes, which on the default "
@@ -203,6 +205,7 @@ def bench_tv_stack(ps, img, n_iter, n_ref):
     tp, up = _timed(run_plain)
     print(f"tv stack: plain loop {tp:.1f} s", file=sys.stderr, flush=True)
     same_plain = all(torch.equal(ub[:, :, z], up[z]) for z in range(Nz))
+    diff_plain = max(float((ub[:, :, z] - up[z]).abs().max()) for z in range(Nz))
     del up
     tr, ur = _timed(run_ref)
     print(f"tv stack: reference-structured loop {tr:.1f} s", file=sys.stderr, flush=True)
@@ -222,6 +225,11 @@ def bench_tv_stack(ps, img, n_iter, n_ref):
             "ms_per_iteration_plain": round(tp / n_iter * 1e3, 2),
             "ms_per_iteration_reference_structured": round(tr / n_ref * 1e3, 2),
             "bitwise_batched_vs_plain": same_plain,
+            "max_abs_diff_batched_vs_plain": diff_plain,
+            "note": "each path takes its own automatic launch shapes (the per-slice calls sum "
+                    "a 2048^2 slice's backprojection over 8 angle bands, the batch plan does "
+                    "not), so the sums are ordered differently; under one pinned shape the two "
+                    "are bitwise equal (tests/test_gpu_configs.py)",
             f"max_abs_diff_batched_vs_reference_structured_at_{n_ref}": diff_ref}
 
 
@@ -382,6 +390,21 @@ def roofline(workload, direction, seconds, work, elem=4):
     out["algorithmic"] = {"bytes_per_launch": algo, "achieved": round(algo / seconds / 1e9, 1),
                           "unit": "GB/s", "of_hbm_peak": round(algo / seconds / HBM_PEAK, 3),
                           "note": "2 taps x element size per pixel-angle; taps come from LDS/L1/L2"}
+    return out
+
+
+def launch_shapes(lib, fan, N, Ns, Na, Nz):
+    """The launch shapes the library takes for the stack (gt_projection_shape)."""
+    import ctypes
+
+    out = {}
+    for adjoint, name, keys in ((0, "forward", ("bins_per_lane", "row_bands", "pixels_per_load",
+                                                "compacted")),
+                                (1, "backprojection", ("pixels_per_lane", "angle_bands",
+                                                       "staged_angles", "tile_width"))):
+        shape = (ctypes.c_int * 4)()
+        if lib.gt_projection_shape(int(fan), adjoint, N, N, Ns, Na, Nz, shape) == 0:
+            out[name] = dict(zip(keys, list(shape)))
     return out
 
 
@@ -557,7 +580,7 @@ def main():
     from gratopy_amd import autotune
 
     flags = {k: getattr(args, k) for k in ("fwd_group", "back_rows", "fwd_bands", "back_bands",
-                                           "back_stage", "back_tile", "fwd_chunk")}
+                                           "back_stage", "back_tile", "fwd_chunk", "fwd_compact")}
     fixed = any(v is not None for v in flags.values())
     if fixed:  # the given knobs; the others keep their values; no automatic choice
         autotune.set_variant(**flags)
@@ -645,11 +668,7 @@ def main():
         "config": {"workload": args.workload, "slices": Nz, "image": [N, N], "angles": Na,
                    "detectors": Ns, "RE": RE, "R": R, "image_width": iw, "detector_width": dw,
                    "order": "F"},
-        "tuning": {"fwd_group": lib.gt_tune_get(0), "fwd_bands": lib.gt_tune_get(2),
-                   "back_rows": lib.gt_tune_get(1), "back_bands": lib.gt_tune_get(3),
-                   "back_stage": lib.gt_tune_get(4), "back_tile": lib.gt_tune_get(5),
-                   "fwd_chunk": lib.gt_tune_get(6), "automatic": lib.gt_tune_get(7),
-                   "source": tuning_source,
+        "tuning": {**launch_shapes(lib, fan, N, Ns, Na, Nz), "source": tuning_source,
                    "trial_ms": None if tuned is None else {"fwd": tuned["fwd_ms"], "back": tuned["back_ms"]}},
         "fwd_ms": round(fwd_med * 1e3, 3),
         "back_ms": round(back_med * 1e3, 3),

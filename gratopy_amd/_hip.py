@@ -44,6 +44,7 @@ PROTOTYPES: dict[str, tuple] = {
     "gt_tune_get": (_i, [_i]),
     "gt_last_error": (_sz, [ctypes.c_char_p, _sz]),
     "gt_projection_workspace_bytes": (_sz, [_i, _i, _i, _i, _i, _i, _i, _i, _i]),
+    "gt_projection_shape": (_i, [_i, _i, _i, _i, _i, _i, _i, ctypes.POINTER(_i)]),
     "gt_radon_fwd": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _vp, _sz]),
     "gt_radon_back": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _vp, _sz]),
     "gt_fan_fwd": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _vp, _sz]),

@@ -34,10 +34,10 @@ from . import _hip
 # knob name -> GT_TUNE_* key of the C ABI; "automatic" (GT_TUNE_AUTO) = 1 lets
 # every call pick its launch shape from its size and ignore the other knobs
 KNOBS = {"fwd_group": 0, "back_rows": 1, "fwd_bands": 2, "back_bands": 3, "back_stage": 4,
-         "back_tile": 5, "fwd_chunk": 6, "automatic": 7}
+         "back_tile": 5, "fwd_chunk": 6, "automatic": 7, "fwd_compact": 8}
 # the library's values at load time
 DEFAULTS = {"fwd_group": 4, "back_rows": 1, "fwd_bands": 1, "back_bands": 1, "back_stage": 0,
-            "back_tile": 16, "fwd_chunk": 1, "automatic": 1}
+            "back_tile": 16, "fwd_chunk": 1, "automatic": 1, "fwd_compact": 1}
 
 FWD_GROUPS = (1, 2, 4, 8, 16)
 BACK_ROWS = (1, 2, 4, 8)
@@ -47,10 +47,12 @@ BACK_STAGES = (0, 8, 16)
 BACK_TILES = (16, 32)
 # forward variants; bands and chunked reads need bin groups; 32 bins per lane
 # come with chunked reads only
-FWD_VARIANTS = ({"fwd_group": 1, "fwd_bands": 1, "fwd_chunk": 1},) + tuple(
-    {"fwd_group": k, "fwd_bands": b, "fwd_chunk": c}
+# (banded ones compacted: only the bin groups that meet a band are launched,
+# bitwise the same result)
+FWD_VARIANTS = ({"fwd_group": 1, "fwd_bands": 1, "fwd_chunk": 1, "fwd_compact": 1},) + tuple(
+    {"fwd_group": k, "fwd_bands": b, "fwd_chunk": c, "fwd_compact": 1}
     for k in FWD_GROUPS if k > 1 for b in FWD_BANDS for c in FWD_CHUNKS) + tuple(
-    {"fwd_group": 32, "fwd_bands": b, "fwd_chunk": 4} for b in FWD_BANDS)
+    {"fwd_group": 32, "fwd_bands": b, "fwd_chunk": 4, "fwd_compact": 1} for b in FWD_BANDS)
 # backprojection variants: unstaged (with or without angle bands), staged, and
 # staged in chunks of 16 angles over 8 angle bands
 BACK_VARIANTS = tuple(

@@ -30,6 +30,7 @@ struct Tuning {
   int back_stage = 0;  // angles per LDS staging chunk of the backprojection (0 = unstaged, 8, 16)
   int back_tile = 16;  // lanes along x of a staged backprojection tile (16, 32)
   int fwd_chunk = 1;   // 4: bin-group strip rows read 16 bytes per load (1 = per pixel)
+  int fwd_compact = 1;  // 1: banded forward launches only the bin groups that meet the band
   int automatic = 1;   // 1: each call picks its launch shape from its size; 0: the knobs above
 };
 Tuning &tuning();

@@ -408,15 +408,20 @@ __device__ __forceinline__ void radon_fwd_ray(const Job<real> &J, int s, int a, 
   store_fwd(J, s, a, z, acc * (real)delta_x * (real)delta_x / (real)delta_xi);
 }
 
-// Unscaled sum of one fan ray (fanbeam.cl:113-250).
+__device__ __forceinline__ void band_rows(int Nyy, int index, int count, int &ylo, int &yhi);
+
+// Unscaled sum of one fan ray (fanbeam.cl:113-250); with band_count > 1 only
+// over the rows of band band_index of its frame.
 template <typename real>
-__device__ __forceinline__ real fan_ray_sum(const Job<real> &J, int s, int a, int z) {
+__device__ __forceinline__ real fan_ray_sum(const Job<real> &J, int s, int a, int z,
+                                            int band_index = 0, int band_count = 1) {
   using A = Arith<real>;
   const FanStrip<real> F = fan_strip(J, a, z, (real)s - (real)1, (real)s + (real)1);
   const real smd = (real)s - J.geo[5];
   int ylo, yhi;
   fan_rows((double)F.qx, (double)F.qy, (double)F.dpx, (double)F.dmx, (double)F.midx,
            (double)F.midy, F.Nxx, F.Nyy, ylo, yhi);
+  if (band_count > 1) band_rows(F.Nyy, band_index, band_count, ylo, yhi);
   const real xcap = (real)(F.Nxx - 1) - F.midx;
   real acc = 0;
   for (int y = ylo; y <= yhi; ++y) {
@@ -529,8 +534,8 @@ __device__ __forceinline__ void fan_fwd_group(const Job<real> &J, int s0, int a,
       if (s >= J.Ns) break;
       if (!BANDED)
         fan_store(J, s, a, z, fan_ray_sum(J, s, a, z));
-      else  // the whole ray goes to band 0
-        *band_slot(J, B, s, a, z) = B.index == 0 ? fan_ray_sum(J, s, a, z) : (real)0;
+      else  // the ray's rows in this band, as for the regular groups
+        *band_slot(J, B, s, a, z) = fan_ray_sum(J, s, a, z, B.index, B.count);
     }
     return;
   }
@@ -685,9 +690,28 @@ __device__ __forceinline__ void radon_fwd_group(const Job<real> &J, int s0, int 
 // and epilogue of the unbanded bodies.
 template <typename real>
 __device__ __forceinline__ void band_reduce_entry(const Job<real> &J, const real *__restrict__ part,
-                                                  int count, int s, int a, int z) {
+                                                  int count, int s, int a, int z,
+                                                  const int2 *__restrict__ range = nullptr,
+                                                  int K = 1) {
   const int64_t stride = (int64_t)J.Nz * J.Na * J.Ns;
   const real *p = part + ((int64_t)z * J.Na + a) * J.Ns + s;
+  if (range) {
+    // compacted launch: band b wrote the groups range[b * Na + a] only; the
+    // others are exact zeros there (no strip pixel in the band), added as
+    // such so that the sum is the uncompacted one bit for bit
+    const int g = s / K;
+    real acc = 0;
+    for (int b = 0; b < count; ++b) {
+      const int2 r = range[(int64_t)b * J.Na + a];
+      const real v = (g >= r.x && g < r.y) ? p[b * stride] : (real)0;
+      acc = b == 0 ? v : acc + v;
+    }
+    if (J.fan)
+      fan_store(J, s, a, z, acc);
+    else
+      store_fwd(J, s, a, z, acc * radon_scale(J));
+    return;
+  }
   real acc = p[0];
   for (int b = 1; b < count; ++b) acc += p[b * stride];
   if (J.fan)

@@ -95,11 +95,191 @@ __global__ __launch_bounds__(256) void fwd_band_kernel(Job<real> J, real *part, 
     radon_fwd_group<real, K, true, CH>(J, g * K, a, blockIdx.y, acc, B);
 }
 
+// ---- compacted banded forward ----------------------------------------------
+// In a row band only some bin groups of an angle have strip pixels: with the
+// parallel beam's detector spanning the image diagonal a third of the groups
+// miss the image, and at oblique angles a band's shadow on the detector is
+// narrower still.  A prologue lists, per band, the groups that can meet it,
+// and the projection launch runs only those, packed into full waves.
+struct FwdPlan {
+  int2 *range;   // [band][angle]: active groups [x, y)
+  int2 *counts;  // [band]: x = wave-table entries used (a multiple of 4), y = packed lanes used
+  int2 *waves;   // [band][wcap]: (angle, first group) of a wave of 64 groups, or (-1, 0)
+  int2 *tail;    // [band][tcap]: (angle, group) of one lane
+  int64_t wcap, tcap;
+};
+
+// Capacities of a plan: whole columns of 64 active groups per angle, in quads
+// of angles, and up to 63 leftover groups per angle.
+static void plan_caps(int Ns, int Na, int K, int64_t &wcap, int64_t &tcap) {
+  const int64_t groups = (Ns + K - 1) / K;
+  wcap = 4 * (int64_t)((Na + 3) / 4) * (groups / 64);
+  tcap = (int64_t)Na * std::min<int64_t>(63, groups);
+}
+
+static size_t plan_bytes(int Ns, int Na, int K, int nb) {
+  int64_t wcap, tcap;
+  plan_caps(Ns, Na, K, wcap, tcap);
+  return align_up(sizeof(int2) * (size_t)nb * ((size_t)Na + 1 + wcap + tcap));
+}
+
+// Groups of angle a whose union strip (bins gK - 1 .. gK + K) can meet a
+// pixel of row band b.  The detector coordinate of a pixel centre is linear
+// (parallel beam) or linear-fractional with a denominator of one sign over
+// the image (fanbeam: the source lies outside it), so over the band's
+// rectangle it takes its extremes at the corners.  Fanbeam groups pick their
+// frame (rows along y or along x) from their own rays, so both frames' bands
+// are covered.  Two bins of margin cover the projection kernels' rounding;
+// non-finite coordinates give every group.
+template <typename real>
+__device__ int2 band_group_range(const Job<real> &J, int K, int groups, int b, int nb, int a) {
+  const real *o = J.ofs + 8 * (int64_t)a;
+  double lo = INFINITY, hi = -INFINITY;
+  bool bad = false, any = false;
+  auto corner = [&](double x, double y) {
+    double s;
+    if (J.fan) {
+      const double ux = x - (double)J.geo[3] - (double)o[2], uy = y - (double)J.geo[4] - (double)o[3];
+      const double wx = (double)o[4] - (double)o[2], wy = (double)o[5] - (double)o[3];
+      s = (double)J.geo[5] + (wx * uy - wy * ux) / ((double)o[1] * ux - (double)o[0] * uy);
+    } else {
+      s = (double)o[0] * x + (double)o[1] * y + (double)o[2];
+    }
+    bad = bad || !(fabs(s) < 1e300);
+    lo = fmin(lo, s);
+    hi = fmax(hi, s);
+  };
+  // frame with rows along y (band over y) and, where it applies, along x
+  const bool rows_y = J.fan || o[5] == (real)0, rows_x = J.fan || o[5] != (real)0;
+  if (rows_y) {
+    const int r0 = (int)((int64_t)J.Ny * b / nb), r1 = (int)((int64_t)J.Ny * (b + 1) / nb) - 1;
+    if (r0 <= r1) {
+      any = true;
+      corner(0, r0), corner(J.Nx - 1, r0), corner(0, r1), corner(J.Nx - 1, r1);
+    }
+  }
+  if (rows_x) {
+    const int r0 = (int)((int64_t)J.Nx * b / nb), r1 = (int)((int64_t)J.Nx * (b + 1) / nb) - 1;
+    if (r0 <= r1) {
+      any = true;
+      corner(r0, 0), corner(r1, 0), corner(r0, J.Ny - 1), corner(r1, J.Ny - 1);
+    }
+  }
+  if (!any) return make_int2(0, 0);
+  if (bad) return make_int2(0, groups);
+  const double m = 2.0;
+  const double g0 = fmax(floor((lo - m) / K) - 1.0, 0.0);
+  const double g1 = fmin(floor((hi + m) / K) + 2.0, (double)groups);
+  return g0 < g1 ? make_int2((int)g0, (int)g1) : make_int2(0, 0);
+}
+
+// Exclusive prefix sum of v over a 256-thread block; *total = the block's sum.
+__device__ inline int block_scan_256(int v, int *total) {
+  __shared__ int wt[4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wt[w] = x;
+  __syncthreads();
+  int off = 0;
+  for (int i = 0; i < w; ++i) off += wt[i];
+  *total = wt[0] + wt[1] + wt[2] + wt[3];
+  __syncthreads();
+  return off + x - v;
+}
+
+// One block per row band: the active range of every angle, then the wave
+// table -- per quad of angles, column c of each of its 4 angles that has it
+// (entry 4 (quad offset + c) + j), so a block's 4 waves walk the same column
+// of 4 adjacent angles -- and the leftover groups (count mod 64) of all
+// angles packed in angle order.
+template <typename real>
+__global__ __launch_bounds__(256) void fwd_plan_kernel(Job<real> J, int K, int nb, FwdPlan P) {
+  const int b = blockIdx.x, Na = J.Na;
+  const int groups = (J.Ns + K - 1) / K;
+  int2 *range = P.range + (int64_t)b * Na;
+  int2 *waves = P.waves + (int64_t)b * P.wcap;
+  int2 *tail = P.tail + (int64_t)b * P.tcap;
+  for (int a = threadIdx.x; a < Na; a += 256) range[a] = band_group_range(J, K, groups, b, nb, a);
+  __syncthreads();
+  const int nq = (Na + 3) / 4;
+  int used = 0, total;
+  for (int base = 0; base < nq; base += 256) {
+    const int q = base + threadIdx.x;
+    int cols[4] = {0, 0, 0, 0}, mc = 0;
+    if (q < nq)
+      for (int j = 0; j < 4; ++j)
+        if (4 * q + j < Na) {
+          const int2 r = range[4 * q + j];
+          cols[j] = (r.y - r.x) / 64;
+          mc = cols[j] > mc ? cols[j] : mc;
+        }
+    const int off = used + block_scan_256(mc, &total);
+    for (int c = 0; c < mc; ++c)
+      for (int j = 0; j < 4; ++j) {
+        const int a = 4 * q + j;
+        waves[4 * (int64_t)(off + c) + j] =
+            c < cols[j] ? make_int2(a, range[a].x + 64 * c) : make_int2(-1, 0);
+      }
+    used += total;
+  }
+  const int entries = 4 * used;
+  used = 0;
+  for (int base = 0; base < Na; base += 256) {
+    const int a = base + threadIdx.x;
+    int n = 0, g0 = 0;
+    if (a < Na) {
+      const int2 r = range[a];
+      n = (r.y - r.x) % 64;
+      g0 = r.y - n;
+    }
+    const int off = used + block_scan_256(n, &total);
+    for (int j = 0; j < n; ++j) tail[(int64_t)off + j] = make_int2(a, g0 + j);
+    used += total;
+  }
+  if (threadIdx.x == 0) P.counts[b] = make_int2(entries, used);
+}
+
+// Banded forward over a plan: block blk of band b is a wave-table block
+// while blk < entries / 4 (wave y of the block = entry 4 blk + y), then a
+// block of 256 packed lanes.
+template <typename real, bool FAN, int K, int CH>
+__global__ __launch_bounds__(256) void fwd_band_compact_kernel(Job<real> J, real *part, int nbands,
+                                                               FwdPlan P) {
+  __shared__ real slots[Slots<K>::kCount * 256];
+  const int band = blockIdx.x % nbands;
+  const int blk = blockIdx.x / nbands;
+  const int2 cnt = P.counts[band];
+  const int fb = cnt.x >> 2;
+  int a, g;
+  if (blk < fb) {
+    const int2 e = P.waves[(int64_t)band * P.wcap + 4 * blk + threadIdx.y];
+    if (e.x < 0) return;
+    a = e.x;
+    g = e.y + threadIdx.x;
+  } else {
+    const int64_t k = (int64_t)(blk - fb) * 256 + threadIdx.y * 64 + threadIdx.x;
+    if (k >= cnt.y) return;
+    const int2 e = P.tail[(int64_t)band * P.tcap + k];
+    a = e.x;
+    g = e.y;
+  }
+  real *acc = slots + threadIdx.y * 64 + threadIdx.x;
+  const Band<real> B{band, nbands, part};
+  if (FAN)
+    fan_fwd_group<real, K, true, CH>(J, g * K, a, blockIdx.y, acc, B);
+  else
+    radon_fwd_group<real, K, true, CH>(J, g * K, a, blockIdx.y, acc, B);
+}
+
 template <typename real>
 __global__ __launch_bounds__(256) void band_reduce_kernel(Job<real> J, const real *part,
-                                                          int nbands) {
+                                                          int nbands, const int2 *range, int K) {
   const int s = blockIdx.x * 256 + threadIdx.x;
-  if (s < J.Ns) band_reduce_entry(J, part, nbands, s, blockIdx.y, blockIdx.z);
+  if (s < J.Ns) band_reduce_entry(J, part, nbands, s, blockIdx.y, blockIdx.z, range, K);
 }
 
 // Backprojection prologue, block (j block, angle, slice): the zero-padded,
@@ -352,6 +532,7 @@ static size_t elem_size(int dtype) { return dtype == GT_FLOAT64 ? 8 : 4; }
 // run the chunked forward and the LDS-staged backprojection.
 struct FwdShape {
   int K, bands, chunk;  // bands and chunked reads need bin groups (K > 1)
+  bool compact;         // banded: launch only the groups that meet each band (FwdPlan)
 };
 struct BackShape {
   int RY, bands, stage, tile;  // stage > 0: LDS-staged, with or without angle bands
@@ -365,12 +546,13 @@ static FwdShape fwd_shape(const Layout &L) {
     const bool g = t.fwd_group > 1;
     // K = 32 exists with chunked reads only
     const int chunk = g ? (t.fwd_group == 32 ? 4 : t.fwd_chunk) : 1;
-    return {t.fwd_group, g ? t.fwd_bands : 1, chunk};
+    const int bands = g ? t.fwd_bands : 1;
+    return {t.fwd_group, bands, chunk, bands > 1 && t.fwd_compact != 0};
   }
   const double w = work_of(L);
   // tiny calls: no row-padded copies (one image copy fewer to launch); the
   // row bands stay, without them a tiny launch has a handful of blocks
-  if (w < 2e6 || (L.fan && w < 5e7)) return {4, 8, 1};
+  if (w < 2e6 || (L.fan && w < 5e7)) return {4, 8, 1, false};
   // 16 bins per lane pay off for the fanbeam's large stacks; parallel-beam
   // strips are straighter and keep 8 (99 ms against 109 ms on the 32 x 2048^2
   // Radon stack)
@@ -378,7 +560,9 @@ static FwdShape fwd_shape(const Layout &L) {
   // 32 row bands where 8 would launch fewer than 4096 blocks (a single slice
   // leaves the GPU part idle), with at least 16 rows per band
   const bool more = group_blocks(L.Ns, L.Na, K) * 8 * L.Nz < 4096 && std::min(L.Nx, L.Ny) >= 512;
-  return {K, more ? 32 : 8, 4};
+  // compaction pays once the prologue's few microseconds are small against
+  // the projection
+  return {K, more ? 32 : 8, 4, w >= 1e9};
 }
 
 static BackShape back_shape(const Layout &L) {
@@ -409,6 +593,7 @@ static size_t fwd_ws(int dtype, const Layout &L, int img_order) {
     if (img_order != GT_ORDER_F) n += img;                  // x-fastest copy
     if (!(img_order == GT_ORDER_C && L.Nz == 1)) n += img;  // y-fastest copy
   }
+  if (sh.compact) n += plan_bytes(L.Ns, L.Na, sh.K, sh.bands);               // group plan
   if (sh.bands > 1) n += align_up(e * (size_t)sh.bands * L.Ns * L.Na * L.Nz);  // band partials
   return n;
 }
@@ -441,12 +626,27 @@ static Job<real> make_job(bool fan, const Layout &L, const real *ofs, const real
 
 // K = 32 comes with chunked reads (16-byte strip loads) only.
 template <typename real, bool FAN, int K>
-static int launch_fwd_group(hipStream_t st, const Job<real> &J, int nb, bool chunk, real *part) {
+static int launch_fwd_group(hipStream_t st, const Job<real> &J, int nb, bool chunk, real *part,
+                            const FwdPlan *plan) {
   constexpr int CH = 16 / sizeof(real);
+  const dim3 block(64, 4);
+  if (plan) {
+    const int64_t per_band = plan->wcap / 4 + (plan->tcap + 255) / 256;
+    GT_REQUIRE(per_band * nb < ((int64_t)1 << 31), "too many detector groups");
+    hipLaunchKernelGGL(fwd_plan_kernel<real>, dim3(nb), dim3(256), 0, st, J, K, nb, *plan);
+    const dim3 grid((unsigned)(per_band * nb), J.Nz);
+    if (chunk)
+      hipLaunchKernelGGL((fwd_band_compact_kernel<real, FAN, K, CH>), grid, block, 0, st, J, part,
+                         nb, *plan);
+    else if constexpr (K <= 16)
+      hipLaunchKernelGGL((fwd_band_compact_kernel<real, FAN, K, 1>), grid, block, 0, st, J, part,
+                         nb, *plan);
+    return GT_OK;
+  }
   const int64_t blocks = group_blocks(J.Ns, J.Na, K);
   GT_REQUIRE(blocks * 256 < ((int64_t)1 << 31) && blocks * nb < ((int64_t)1 << 31),
              "too many detector groups");
-  const dim3 grid((unsigned)(blocks * nb), J.Nz), block(64, 4);
+  const dim3 grid((unsigned)(blocks * nb), J.Nz);
   if (chunk) {
     if (nb > 1)
       hipLaunchKernelGGL((fwd_band_kernel<real, FAN, K, CH>), grid, block, 0, st, J, part, nb);
@@ -511,10 +711,22 @@ static int fwd_impl(hipStream_t st, real *sino, const real *img, const real *ofs
   const int K = sh.K, nb = sh.bands;
   const bool chunk = sh.chunk > 1;
   real *part = nullptr;
+  FwdPlan plan{}, *pp = nullptr;
   if (nb > 1) {
-    // the band partials are the last part of the workspace (after the image copies)
+    // the band partials are the last part of the workspace, the group plan
+    // (compacted launches) comes before them, after the image copies
     const size_t part_bytes = align_up(sizeof(real) * (size_t)nb * L.Ns * L.Na * L.Nz);
     part = reinterpret_cast<real *>(static_cast<char *>(ws) + (need - part_bytes));
+    if (sh.compact) {
+      int2 *q = reinterpret_cast<int2 *>(static_cast<char *>(ws) + (need - part_bytes -
+                                                                     plan_bytes(L.Ns, L.Na, K, nb)));
+      plan_caps(L.Ns, L.Na, K, plan.wcap, plan.tcap);
+      plan.range = q;
+      plan.counts = q + (int64_t)nb * L.Na;
+      plan.waves = plan.counts + nb;
+      plan.tail = plan.waves + nb * plan.wcap;
+      pp = &plan;
+    }
   }
   if (K == 1) {
     const dim3 grid((L.Ns + 63) / 64, (L.Na + 3) / 4, L.Nz);
@@ -522,20 +734,20 @@ static int fwd_impl(hipStream_t st, real *sino, const real *img, const real *ofs
   } else {
     int rc;
     if (K == 32)
-      rc = launch_fwd_group<real, FAN, 32>(st, J, nb, chunk, part);
+      rc = launch_fwd_group<real, FAN, 32>(st, J, nb, chunk, part, pp);
     else if (K == 16)
-      rc = launch_fwd_group<real, FAN, 16>(st, J, nb, chunk, part);
+      rc = launch_fwd_group<real, FAN, 16>(st, J, nb, chunk, part, pp);
     else if (K == 8)
-      rc = launch_fwd_group<real, FAN, 8>(st, J, nb, chunk, part);
+      rc = launch_fwd_group<real, FAN, 8>(st, J, nb, chunk, part, pp);
     else if (K == 4)
-      rc = launch_fwd_group<real, FAN, 4>(st, J, nb, chunk, part);
+      rc = launch_fwd_group<real, FAN, 4>(st, J, nb, chunk, part, pp);
     else
-      rc = launch_fwd_group<real, FAN, 2>(st, J, nb, chunk, part);
+      rc = launch_fwd_group<real, FAN, 2>(st, J, nb, chunk, part, pp);
     if (rc) return rc;
   }
   if (nb > 1)
     hipLaunchKernelGGL(band_reduce_kernel<real>, dim3((L.Ns + 255) / 256, L.Na, L.Nz), dim3(256),
-                       0, st, J, part, nb);
+                       0, st, J, part, nb, pp ? (const int2 *)plan.range : nullptr, K);
   return check_launch(FAN ? "fan forward projection" : "radon forward projection");
 }
 
@@ -594,6 +806,22 @@ extern "C" size_t gt_projection_workspace_bytes(int geometry, int adjoint, int d
   if (Nx <= 0 || Ny <= 0 || Ns <= 0 || Na <= 0 || Nz <= 0) return 0;
   const Layout L{Nx, Ny, Ns, Na, Nz, geometry == GT_GEOMETRY_FAN};
   return adjoint ? back_ws(dtype, L) : fwd_ws(dtype, L, in_order);
+}
+
+extern "C" int gt_projection_shape(int geometry, int adjoint, int Nx, int Ny, int Ns, int Na,
+                                   int Nz, int *shape) {
+  if (!shape) return fail(GT_EINVAL, "shape pointer is NULL");
+  const Layout L{Nx, Ny, Ns, Na, Nz, geometry == GT_GEOMETRY_FAN};
+  int rc = check_sizes(L);
+  if (rc) return rc;
+  if (adjoint) {
+    const BackShape b = back_shape(L);
+    shape[0] = b.RY, shape[1] = b.bands, shape[2] = b.stage, shape[3] = b.tile;
+  } else {
+    const FwdShape f = fwd_shape(L);
+    shape[0] = f.K, shape[1] = f.bands, shape[2] = f.chunk, shape[3] = f.compact ? 1 : 0;
+  }
+  return GT_OK;
 }
 
 #define GT_DISPATCH(dtype, CALL_F32, CALL_F64) \

@@ -68,6 +68,10 @@ extern "C" int gt_tune_set(int key, int value) {
         return gt::fail(GT_EINVAL, "forward chunk must be 1 or 4");
       t.fwd_chunk = value;
       return GT_OK;
+    case GT_TUNE_FWD_COMPACT:
+      if (value != 0 && value != 1) return gt::fail(GT_EINVAL, "forward compaction: 0 or 1");
+      t.fwd_compact = value;
+      return GT_OK;
     case GT_TUNE_BACK_TILE:
       if (value != 16 && value != 32)
         return gt::fail(GT_EINVAL, "staged backprojection tile width must be 16 or 32");
@@ -97,6 +101,8 @@ extern "C" int gt_tune_get(int key) {
       return t.fwd_chunk;
     case GT_TUNE_AUTO:
       return t.automatic;
+    case GT_TUNE_FWD_COMPACT:
+      return t.fwd_compact;
     default:
       return -1;
   }

@@ -68,6 +68,11 @@ int gt_version(void);
  *                      load (4 float32 or 2 float64 pixels) from row-padded copies of
  *                      the image (the workspace then holds two padded copies); 1 = one
  *                      pixel per load.
+ *   GT_TUNE_FWD_COMPACT: 1 (the default) = a banded forward launches, per row band and
+ *                      angle, only the bin groups whose strips can meet the band (a
+ *                      prologue kernel lists them in the workspace); 0 = every group in
+ *                      every band.  The results are the same bit for bit.  Batch plans do
+ *                      not compact.
  *   GT_TUNE_AUTO:      1 (the default) = every call (and batch plan) picks its launch
  *                      shape from its geometry and sizes and the knobs above are
  *                      ignored; 0 = the knobs above apply.  Workspace sizes follow the
@@ -80,7 +85,8 @@ enum {
   GT_TUNE_BACK_STAGE = 4,
   GT_TUNE_BACK_TILE = 5,
   GT_TUNE_FWD_CHUNK = 6,
-  GT_TUNE_AUTO = 7
+  GT_TUNE_AUTO = 7,
+  GT_TUNE_FWD_COMPACT = 8
 };
 int gt_tune_set(int key, int value);
 int gt_tune_get(int key);
@@ -97,6 +103,15 @@ size_t gt_last_error(char *buf, size_t n);
  * workspace than it needs returns GT_ENOSPC. */
 size_t gt_projection_workspace_bytes(int geometry, int adjoint, int dtype, int Nx, int Ny,
                                      int Ns, int Na, int Nz, int in_order);
+
+/* The launch shape a projection call with these sizes takes under the current
+ * knobs (the automatic choice when GT_TUNE_AUTO = 1): forward: shape[0..3] =
+ * detector bins per lane, row bands, pixels per strip load, compacted (0/1);
+ * backprojection: shape[0..3] = pixels per lane, angle bands, angles per LDS
+ * staging chunk (0 = unstaged), staged tile width.  No reference counterpart
+ * (it describes this library's launches).  GT_OK, or GT_EINVAL for invalid sizes. */
+int gt_projection_shape(int geometry, int adjoint, int Nx, int Ny, int Ns, int Na, int Nz,
+                        int *shape);
 
 /* Parallel-beam forward projection sino = R img.
  * Replaces kernel radon_<t>_<sino_order><img_order> (gratopy/radon.cl:66-161),

@@ -86,7 +86,7 @@ def test_tuning_knobs_accept_their_values_and_reject_others(lib):
     before = autotune.current_variant()
     valid = {"fwd_group": (1, 2, 4, 8, 16, 32), "back_rows": (1, 2, 4, 8),
              "fwd_bands": (1, 2, 4, 8, 16, 32), "back_bands": (1, 2, 4, 8), "back_stage": (0, 8, 16),
-             "back_tile": (16, 32), "fwd_chunk": (1, 4), "automatic": (0, 1)}
+             "back_tile": (16, 32), "fwd_chunk": (1, 4), "automatic": (0, 1), "fwd_compact": (0, 1)}
     try:
         assert set(valid) == set(autotune.KNOBS)
         for name, values in valid.items():
@@ -108,6 +108,12 @@ def test_tuning_knobs_accept_their_values_and_reject_others(lib):
         # 32 bins per lane always read 16 bytes per load (padded copies)
         autotune.set_variant(fwd_group=32)
         assert lib.gt_projection_workspace_bytes(1, 0, 0, 100, 60, 80, 30, 1, 0) == padded
+        # a compacted banded forward adds its group plan to the band partials
+        autotune.set_variant(fwd_group=8, fwd_bands=8, fwd_chunk=4, fwd_compact=0)
+        plain = lib.gt_projection_workspace_bytes(1, 0, 0, 100, 60, 80, 30, 1, 0)
+        assert plain >= padded + 4 * 8 * 80 * 30
+        autotune.set_variant(fwd_compact=1)
+        assert lib.gt_projection_workspace_bytes(1, 0, 0, 100, 60, 80, 30, 1, 0) >= plain + 8 * 8 * 30
     finally:
         autotune.set_variant(**before)
 
@@ -191,3 +197,32 @@ def test_noalloc_rejects_calls_beyond_its_array(lib):
     assert rc == _hip.GT_ENOSPC and "no-allocation" in _hip.last_error()
     rc = lib.gt_projection_noalloc(None, 1, 2, 1, *p, 64, 64, 64, 64, 1, 0, 0)
     assert rc == _hip.GT_EINVAL and "adjoint" in _hip.last_error()
+
+
+def test_projection_shape_reports_the_automatic_choice(lib):
+    from gratopy_amd import autotune
+
+    before = autotune.current_variant()
+    try:
+        autotune.set_variant(automatic=1)
+        shape = (ctypes.c_int * 4)()
+        # the benchmark stacks: 32 x 2048^2, 1440 angles, 2560 detectors
+        assert lib.gt_projection_shape(1, 0, 2048, 2048, 2560, 1440, 32, shape) == _hip.GT_OK
+        assert list(shape) == [16, 8, 4, 1]
+        assert lib.gt_projection_shape(0, 0, 2048, 2048, 2560, 1440, 32, shape) == _hip.GT_OK
+        assert list(shape) == [8, 8, 4, 1]
+        assert lib.gt_projection_shape(1, 1, 2048, 2048, 2560, 1440, 32, shape) == _hip.GT_OK
+        assert list(shape) == [8, 1, 16, 32]
+        # tiny calls: 4 bins per lane, per-pixel reads, no compaction; unstaged back
+        assert lib.gt_projection_shape(0, 0, 32, 32, 48, 30, 1, shape) == _hip.GT_OK
+        assert list(shape) == [4, 8, 1, 0]
+        assert lib.gt_projection_shape(0, 1, 32, 32, 48, 30, 1, shape) == _hip.GT_OK
+        assert list(shape)[2] == 0
+        # explicit knobs are reported as set
+        autotune.set_variant(fwd_group=2, fwd_bands=32, fwd_chunk=1, fwd_compact=0)
+        assert lib.gt_projection_shape(0, 0, 2048, 2048, 2560, 1440, 32, shape) == _hip.GT_OK
+        assert list(shape) == [2, 32, 1, 0]
+        assert lib.gt_projection_shape(0, 0, 0, 2048, 2560, 1440, 32, shape) == _hip.GT_EINVAL
+        assert lib.gt_projection_shape(0, 0, 8, 8, 8, 8, 1, None) == _hip.GT_EINVAL
+    finally:
+        autotune.set_variant(**before)
