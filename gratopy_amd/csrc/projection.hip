@@ -553,16 +553,19 @@ static FwdShape fwd_shape(const Layout &L) {
   // tiny calls: no row-padded copies (one image copy fewer to launch); the
   // row bands stay, without them a tiny launch has a handful of blocks
   if (w < 2e6 || (L.fan && w < 5e7)) return {4, 8, 1, false};
-  // 16 bins per lane pay off for the fanbeam's large stacks; parallel-beam
-  // strips are straighter and keep 8 (99 ms against 109 ms on the 32 x 2048^2
-  // Radon stack)
-  const int K = (w < 2e9 || !L.fan) ? 8 : 16;
+  const int K = w < 2e9 ? 8 : 16;
   // 32 row bands where 8 would launch fewer than 4096 blocks (a single slice
   // leaves the GPU part idle), with at least 16 rows per band
   const bool more = group_blocks(L.Ns, L.Na, K) * 8 * L.Nz < 4096 && std::min(L.Nx, L.Ny) >= 512;
+  int bands = more ? 32 : 8;
+  // large stacks: 16 bands while their partials stay under 2^31 elements;
+  // a narrower band has a narrower shadow on the detector, so compaction
+  // drops more groups (32 x 2048^2 stacks: fanbeam 87.4 -> 86.0 ms, parallel
+  // beam 88.7 -> 81.3 ms)
+  if (!more && w >= 2e9 && 16.0 * L.Ns * L.Na * L.Nz <= 2147483648.0) bands = 16;
   // compaction pays once the prologue's few microseconds are small against
   // the projection
-  return {K, more ? 32 : 8, 4, w >= 1e9};
+  return {K, bands, 4, w >= 1e9};
 }
 
 static BackShape back_shape(const Layout &L) {
@@ -573,7 +576,7 @@ static BackShape back_shape(const Layout &L) {
   // Staged tiles hold 2048 pixels at either width; angle bands while the
   // tiles of the call fill the GPU fewer than ~3 times (4096 tiles).
   const int bands = (double)L.Nx * L.Ny * L.Nz < 4096.0 * 2048.0 ? 8 : 1;
-  return {8, bands, 16, (L.fan || work_of(L) >= 2e10) ? 32 : 16};
+  return {8, bands, 16, L.fan ? 32 : 16};
 }
 
 // Chunked strip reads (chunk = 4) read 16 bytes per load and need both image

@@ -32,7 +32,8 @@ CASES = [
 ]
 
 VARIANTS = [dict(fwd_group=k, fwd_bands=b, fwd_chunk=c)
-            for k, b, c in ((2, 8, 1), (4, 32, 1), (8, 8, 4), (16, 8, 4), (16, 32, 4), (32, 8, 4))]
+            for k, b, c in ((2, 8, 1), (4, 32, 1), (8, 8, 4), (16, 8, 4), (16, 16, 4), (16, 32, 4),
+                            (32, 8, 4))]
 
 
 @pytest.fixture
