@@ -483,15 +483,29 @@ __device__ __forceinline__ void band_rows(int Nyy, int index, int count, int &yl
   yhi = yhi < r1 ? yhi : r1;
 }
 
+// Lane of a whole-column block (64 groups x 4 adjacent angles): wave y
+// takes groups [16 y, 16 y + 16) of the column for all 4 angles, lane x the
+// angle x / 16 and group 16 y + x % 16.  Returns the angle's index in the
+// block; *goff the group's offset in the column.  A wave then spans 16
+// groups (a quarter of the detector range of 64) at 4 nearly equal angles:
+// its lanes' strips have nearly equal widths per row, so a row costs the
+// wave little more than its lanes' own 16-byte chunks (64 groups of one
+// angle span ~40 % of a fan, whose strip widths differ by a chunk or more).
+__device__ __forceinline__ int column_lane(int &goff) {
+  goff = 16 * threadIdx.y + (threadIdx.x & 15);
+  return threadIdx.x >> 4;
+}
+
 // Lane mapping of the bin-group forward launches (256-lane blocks of 64 x 4;
 // groups = ceil(Ns / K) bin groups per angle).  The whole columns of 64
-// groups come first, one block per (column, 4 angles): its 4 waves walk
-// nearly the same pixels, which the CU's cache then serves once.  The
-// groups left over (groups % 64 of them: 32 of 160 with K = 16 and 2560
-// bins; all of a small slice's) are packed over consecutive angles, lane
-// (a * tail + j) taking group cols*64 + j of angle a, so their waves are
-// full instead of mostly idle and a block holds the same detector edge for
-// a run of adjacent angles.  Returns a (>= Na for lanes past the end).
+// groups come first, one block per (column, 4 angles), lanes by
+// column_lane: its waves walk nearly the same pixels, which the CU's cache
+// then serves once.  The groups left over (groups % 64 of them: 32 of 160
+// with K = 16 and 2560 bins; all of a small slice's) are packed over
+// consecutive angles, lane (a * tail + j) taking group cols*64 + j of angle
+// a, so their waves are full instead of mostly idle and a block holds the
+// same detector edge for a run of adjacent angles.  Returns a (>= Na for
+// lanes past the end).
 template <int K>
 __device__ __forceinline__ int group_lane(int block, int Ns, int Na, int &g) {
   const int groups = (Ns + K - 1) / K;
@@ -499,8 +513,10 @@ __device__ __forceinline__ int group_lane(int block, int Ns, int Na, int &g) {
   const int nfb = cols * ((Na + 3) / 4);
   if (block < nfb) {
     const int ab = block / cols;
-    g = (block - ab * cols) * 64 + threadIdx.x;
-    return ab * 4 + threadIdx.y;
+    int goff;
+    const int j = column_lane(goff);
+    g = (block - ab * cols) * 64 + goff;
+    return ab * 4 + j;
   }
   const int tail = groups - cols * 64;
   const int lin = (block - nfb) * 256 + threadIdx.y * 64 + threadIdx.x;

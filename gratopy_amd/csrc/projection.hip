@@ -193,9 +193,9 @@ __device__ inline int block_scan_256(int v, int *total) {
 
 // One block per row band: the active range of every angle, then the wave
 // table -- per quad of angles, column c of each of its 4 angles that has it
-// (entry 4 (quad offset + c) + j), so a block's 4 waves walk the same column
-// of 4 adjacent angles -- and the leftover groups (count mod 64) of all
-// angles packed in angle order.
+// (entry 4 (quad offset + c) + j), so a block walks the same column of 4
+// adjacent angles -- and the leftover groups (count mod 64) of all angles
+// packed in angle order.
 template <typename real>
 __global__ __launch_bounds__(256) void fwd_plan_kernel(Job<real> J, int K, int nb, FwdPlan P) {
   const int b = blockIdx.x, Na = J.Na;
@@ -244,8 +244,8 @@ __global__ __launch_bounds__(256) void fwd_plan_kernel(Job<real> J, int K, int n
 }
 
 // Banded forward over a plan: block blk of band b is a wave-table block
-// while blk < entries / 4 (wave y of the block = entry 4 blk + y), then a
-// block of 256 packed lanes.
+// while blk < entries / 4 (the 4 angles of its quad: entries 4 blk + j,
+// lanes by column_lane), then a block of 256 packed lanes.
 template <typename real, bool FAN, int K, int CH>
 __global__ __launch_bounds__(256) void fwd_band_compact_kernel(Job<real> J, real *part, int nbands,
                                                                FwdPlan P) {
@@ -256,10 +256,12 @@ __global__ __launch_bounds__(256) void fwd_band_compact_kernel(Job<real> J, real
   const int fb = cnt.x >> 2;
   int a, g;
   if (blk < fb) {
-    const int2 e = P.waves[(int64_t)band * P.wcap + 4 * blk + threadIdx.y];
+    int goff;
+    const int j = column_lane(goff);
+    const int2 e = P.waves[(int64_t)band * P.wcap + 4 * blk + j];
     if (e.x < 0) return;
     a = e.x;
-    g = e.y + threadIdx.x;
+    g = e.y + goff;
   } else {
     const int64_t k = (int64_t)(blk - fb) * 256 + threadIdx.y * 64 + threadIdx.x;
     if (k >= cnt.y) return;
