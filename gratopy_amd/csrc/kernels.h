@@ -483,52 +483,60 @@ __device__ __forceinline__ void band_rows(int Nyy, int index, int count, int &yl
   yhi = yhi < r1 ? yhi : r1;
 }
 
-// Lane of a whole-column block (64 groups x 4 adjacent angles): wave y
-// takes groups [16 y, 16 y + 16) of the column for all 4 angles, lane x the
-// angle x / 16 and group 16 y + x % 16.  Returns the angle's index in the
-// block; *goff the group's offset in the column.  A wave then spans 16
-// groups (a quarter of the detector range of 64) at 4 nearly equal angles:
-// its lanes' strips have nearly equal widths per row, so a row costs the
-// wave little more than its lanes' own 16-byte chunks (64 groups of one
-// angle span ~40 % of a fan, whose strip widths differ by a chunk or more).
+// Whole-column blocks of the bin-group forward cover kColumnGroups adjacent
+// groups of kColumnAngles adjacent angles (256 lanes).
+constexpr int kColumnAngles = 16;
+constexpr int kColumnGroups = 256 / kColumnAngles;
+
+// Lane of a whole-column block: wave y takes groups [4 y, 4 y + 4) of the
+// column for all 16 angles, lane x the angle x / 4 and group 4 y + x % 4.
+// Returns the angle's index in the block; *goff the group's offset in the
+// column.  A wave then spans a few groups at 16 angles 0.25 degrees apart
+// (on the 1440-angle stacks): its lanes' strips have nearly equal widths
+// per row, so a row costs the wave little more than its lanes' own 16-byte
+// chunks (64 groups of one angle span ~40 % of a fan, whose strip widths
+// differ by a chunk or more), and its loads span a narrow part of the row.
 __device__ __forceinline__ int column_lane(int &goff) {
-  goff = 16 * threadIdx.y + (threadIdx.x & 15);
-  return threadIdx.x >> 4;
+  constexpr int LW = 64 / kColumnAngles;
+  goff = LW * threadIdx.y + (threadIdx.x % LW);
+  return threadIdx.x / LW;
 }
 
 // Lane mapping of the bin-group forward launches (256-lane blocks of 64 x 4;
-// groups = ceil(Ns / K) bin groups per angle).  The whole columns of 64
-// groups come first, one block per (column, 4 angles), lanes by
-// column_lane: its waves walk nearly the same pixels, which the CU's cache
-// then serves once.  The groups left over (groups % 64 of them: 32 of 160
-// with K = 16 and 2560 bins; all of a small slice's) are packed over
-// consecutive angles, lane (a * tail + j) taking group cols*64 + j of angle
-// a, so their waves are full instead of mostly idle and a block holds the
-// same detector edge for a run of adjacent angles.  Returns a (>= Na for
-// lanes past the end).
+// groups = ceil(Ns / K) bin groups per angle).  The whole columns of
+// kColumnGroups groups come first, one block per (column, kColumnAngles
+// angles), lanes by column_lane: its waves walk nearly the same pixels,
+// which the CU's cache then serves once.  The groups left over
+// (groups % kColumnGroups of them; all of a small slice's) are packed over
+// consecutive angles, lane (a * tail + j) taking group cols * kColumnGroups
+// + j of angle a, so their waves are full instead of mostly idle and a block
+// holds the same detector edge for a run of adjacent angles.  Returns a (>=
+// Na for lanes past the end).
 template <int K>
 __device__ __forceinline__ int group_lane(int block, int Ns, int Na, int &g) {
+  constexpr int A = kColumnAngles, C = kColumnGroups;
   const int groups = (Ns + K - 1) / K;
-  const int cols = groups / 64;
-  const int nfb = cols * ((Na + 3) / 4);
+  const int cols = groups / C;
+  const int nfb = cols * ((Na + A - 1) / A);
   if (block < nfb) {
     const int ab = block / cols;
     int goff;
     const int j = column_lane(goff);
-    g = (block - ab * cols) * 64 + goff;
-    return ab * 4 + j;
+    g = (block - ab * cols) * C + goff;
+    return ab * A + j;
   }
-  const int tail = groups - cols * 64;
+  const int tail = groups - cols * C;
   const int lin = (block - nfb) * 256 + threadIdx.y * 64 + threadIdx.x;
   const int a = lin / tail;
-  g = cols * 64 + (lin - a * tail);
+  g = cols * C + (lin - a * tail);
   return a;
 }
 
 // Blocks of a bin-group forward launch over one slice (group_lane).
 inline int64_t group_blocks(int Ns, int Na, int K) {
-  const int64_t groups = (Ns + K - 1) / K, cols = groups / 64, tail = groups - cols * 64;
-  return cols * ((Na + 3) / 4) + (Na * tail + 255) / 256;
+  const int64_t groups = (Ns + K - 1) / K, cols = groups / kColumnGroups;
+  const int64_t tail = groups - cols * kColumnGroups;
+  return cols * ((Na + kColumnAngles - 1) / kColumnAngles) + (Na * tail + 255) / 256;
 }
 
 // acc: this lane's slot 0 in an LDS array of Slots<K>::kCount rows of 256.

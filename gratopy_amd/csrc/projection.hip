@@ -103,18 +103,26 @@ __global__ __launch_bounds__(256) void fwd_band_kernel(Job<real> J, real *part, 
 // and the projection launch runs only those, packed into full waves.
 struct FwdPlan {
   int2 *range;   // [band][angle]: active groups [x, y)
-  int2 *counts;  // [band]: x = wave-table entries used (a multiple of 4), y = packed lanes used
-  int2 *waves;   // [band][wcap]: (angle, first group) of a wave of 64 groups, or (-1, 0)
+  int2 *counts;  // [band]: x = column entries used (a multiple of kPlanAngles), y = packed lanes used
+  int2 *waves;   // [band][wcap]: (angle, first group) of one angle's column, or (-1, 0)
   int2 *tail;    // [band][tcap]: (angle, group) of one lane
   int64_t wcap, tcap;
 };
 
-// Capacities of a plan: whole columns of 64 active groups per angle, in quads
-// of angles, and up to 63 leftover groups per angle.
+// A block of the compacted launch walks one column of kPlanCols groups of
+// kPlanAngles adjacent angles (column_lane), as a whole-column block of the
+// full launch does.  Angles per block measured on the 32 x 2048^2 stacks
+// (fanbeam / parallel-beam forward, ms): 4: 78.1 / 72.0, 8: 75.6 / 70.2,
+// 16: 74.4 / 69.8, 32: 75.0 / 69.3, 64: 80.4 / 71.5.
+constexpr int kPlanAngles = kColumnAngles;
+constexpr int kPlanCols = kColumnGroups;
+
+// Capacities of a plan: whole columns of active groups per angle, in sets of
+// kPlanAngles angles, and up to kPlanCols - 1 leftover groups per angle.
 static void plan_caps(int Ns, int Na, int K, int64_t &wcap, int64_t &tcap) {
   const int64_t groups = (Ns + K - 1) / K;
-  wcap = 4 * (int64_t)((Na + 3) / 4) * (groups / 64);
-  tcap = (int64_t)Na * std::min<int64_t>(63, groups);
+  wcap = kPlanAngles * (int64_t)((Na + kPlanAngles - 1) / kPlanAngles) * (groups / kPlanCols);
+  tcap = (int64_t)Na * std::min<int64_t>(kPlanCols - 1, groups);
 }
 
 static size_t plan_bytes(int Ns, int Na, int K, int nb) {
@@ -192,12 +200,13 @@ __device__ inline int block_scan_256(int v, int *total) {
 }
 
 // One block per row band: the active range of every angle, then the wave
-// table -- per quad of angles, column c of each of its 4 angles that has it
-// (entry 4 (quad offset + c) + j), so a block walks the same column of 4
-// adjacent angles -- and the leftover groups (count mod 64) of all angles
-// packed in angle order.
+// table -- per set of kPlanAngles angles, column c of each of its angles that
+// has it (entry kPlanAngles (set offset + c) + j), so a block walks the same
+// column of adjacent angles -- and the leftover groups (count mod kPlanCols)
+// of all angles packed in angle order.
 template <typename real>
 __global__ __launch_bounds__(256) void fwd_plan_kernel(Job<real> J, int K, int nb, FwdPlan P) {
+  constexpr int A = kPlanAngles, C = kPlanCols;
   const int b = blockIdx.x, Na = J.Na;
   const int groups = (J.Ns + K - 1) / K;
   int2 *range = P.range + (int64_t)b * Na;
@@ -205,35 +214,35 @@ __global__ __launch_bounds__(256) void fwd_plan_kernel(Job<real> J, int K, int n
   int2 *tail = P.tail + (int64_t)b * P.tcap;
   for (int a = threadIdx.x; a < Na; a += 256) range[a] = band_group_range(J, K, groups, b, nb, a);
   __syncthreads();
-  const int nq = (Na + 3) / 4;
+  const int nq = (Na + A - 1) / A;
   int used = 0, total;
   for (int base = 0; base < nq; base += 256) {
     const int q = base + threadIdx.x;
-    int cols[4] = {0, 0, 0, 0}, mc = 0;
+    int cols[A] = {}, mc = 0;
     if (q < nq)
-      for (int j = 0; j < 4; ++j)
-        if (4 * q + j < Na) {
-          const int2 r = range[4 * q + j];
-          cols[j] = (r.y - r.x) / 64;
+      for (int j = 0; j < A; ++j)
+        if (A * q + j < Na) {
+          const int2 r = range[A * q + j];
+          cols[j] = (r.y - r.x) / C;
           mc = cols[j] > mc ? cols[j] : mc;
         }
     const int off = used + block_scan_256(mc, &total);
     for (int c = 0; c < mc; ++c)
-      for (int j = 0; j < 4; ++j) {
-        const int a = 4 * q + j;
-        waves[4 * (int64_t)(off + c) + j] =
-            c < cols[j] ? make_int2(a, range[a].x + 64 * c) : make_int2(-1, 0);
+      for (int j = 0; j < A; ++j) {
+        const int a = A * q + j;
+        waves[A * (int64_t)(off + c) + j] =
+            c < cols[j] ? make_int2(a, range[a].x + C * c) : make_int2(-1, 0);
       }
     used += total;
   }
-  const int entries = 4 * used;
+  const int entries = A * used;
   used = 0;
   for (int base = 0; base < Na; base += 256) {
     const int a = base + threadIdx.x;
     int n = 0, g0 = 0;
     if (a < Na) {
       const int2 r = range[a];
-      n = (r.y - r.x) % 64;
+      n = (r.y - r.x) % C;
       g0 = r.y - n;
     }
     const int off = used + block_scan_256(n, &total);
@@ -243,9 +252,9 @@ __global__ __launch_bounds__(256) void fwd_plan_kernel(Job<real> J, int K, int n
   if (threadIdx.x == 0) P.counts[b] = make_int2(entries, used);
 }
 
-// Banded forward over a plan: block blk of band b is a wave-table block
-// while blk < entries / 4 (the 4 angles of its quad: entries 4 blk + j,
-// lanes by column_lane), then a block of 256 packed lanes.
+// Banded forward over a plan: block blk of band b is a column block while
+// blk < entries / kPlanAngles (its angles: entries kPlanAngles blk + j, lanes
+// by column_lane), then a block of 256 packed lanes.
 template <typename real, bool FAN, int K, int CH>
 __global__ __launch_bounds__(256) void fwd_band_compact_kernel(Job<real> J, real *part, int nbands,
                                                                FwdPlan P) {
@@ -253,12 +262,12 @@ __global__ __launch_bounds__(256) void fwd_band_compact_kernel(Job<real> J, real
   const int band = blockIdx.x % nbands;
   const int blk = blockIdx.x / nbands;
   const int2 cnt = P.counts[band];
-  const int fb = cnt.x >> 2;
+  const int fb = cnt.x / kPlanAngles;
   int a, g;
   if (blk < fb) {
     int goff;
     const int j = column_lane(goff);
-    const int2 e = P.waves[(int64_t)band * P.wcap + 4 * blk + j];
+    const int2 e = P.waves[(int64_t)band * P.wcap + kPlanAngles * blk + j];
     if (e.x < 0) return;
     a = e.x;
     g = e.y + goff;
@@ -636,7 +645,7 @@ static int launch_fwd_group(hipStream_t st, const Job<real> &J, int nb, bool chu
   constexpr int CH = 16 / sizeof(real);
   const dim3 block(64, 4);
   if (plan) {
-    const int64_t per_band = plan->wcap / 4 + (plan->tcap + 255) / 256;
+    const int64_t per_band = plan->wcap / kPlanAngles + (plan->tcap + 255) / 256;
     GT_REQUIRE(per_band * nb < ((int64_t)1 << 31), "too many detector groups");
     hipLaunchKernelGGL(fwd_plan_kernel<real>, dim3(nb), dim3(256), 0, st, J, K, nb, *plan);
     const dim3 grid((unsigned)(per_band * nb), J.Nz);
