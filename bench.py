@@ -227,9 +227,9 @@ def bench_tv_stack(ps, img, n_iter, n_ref):
             "bitwise_batched_vs_plain": same_plain,
             "max_abs_diff_batched_vs_plain": diff_plain,
             "note": "each path takes its own automatic launch shapes (the per-slice calls sum "
-                    "a 2048^2 slice's backprojection over 8 angle bands, the batch plan does "
-                    "not), so the sums are ordered differently; under one pinned shape the two "
-                    "are bitwise equal (tests/test_gpu_configs.py)",
+                    "a 2048^2 slice's backprojection over 8 angle bands, the launches over the "
+                    "whole stack do not), so the sums are ordered differently; under one pinned "
+                    "shape the two are bitwise equal (tests/test_gpu_configs.py)",
             f"max_abs_diff_batched_vs_reference_structured_at_{n_ref}": diff_ref}
 
 
