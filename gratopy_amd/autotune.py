@@ -41,7 +41,7 @@ DEFAULTS = {"fwd_group": 4, "back_rows": 1, "fwd_bands": 1, "back_bands": 1, "ba
 
 FWD_GROUPS = (1, 2, 4, 8, 16)
 BACK_ROWS = (1, 2, 4, 8)
-FWD_BANDS = (1, 8, 16, 32)
+FWD_BANDS = (1, 4, 8, 16, 32)
 FWD_CHUNKS = (1, 4)
 BACK_STAGES = (0, 8, 16)
 BACK_TILES = (16, 32)

@@ -565,15 +565,19 @@ static FwdShape fwd_shape(const Layout &L) {
   // row bands stay, without them a tiny launch has a handful of blocks
   if (w < 2e6 || (L.fan && w < 5e7)) return {4, 8, 1, false};
   const int K = w < 2e9 ? 8 : 16;
-  // 32 row bands where 8 would launch fewer than 4096 blocks (a single slice
-  // leaves the GPU part idle), with at least 16 rows per band
-  const bool more = group_blocks(L.Ns, L.Na, K) * 8 * L.Nz < 4096 && std::min(L.Nx, L.Ny) >= 512;
-  int bands = more ? 32 : 8;
-  // large stacks: 16 bands while their partials stay under 2^31 elements;
-  // a narrower band has a narrower shadow on the detector, so compaction
-  // drops more groups (32 x 2048^2 stacks: fanbeam 87.4 -> 86.0 ms, parallel
-  // beam 88.7 -> 81.3 ms)
-  if (!more && w >= 2e9 && 16.0 * L.Ns * L.Na * L.Nz <= 2147483648.0) bands = 16;
+  // Row bands: the fewest of 4, 8, 16, 32 that give the launch at least
+  // 12288 blocks (48 per CU); more than 8 only with at least 16 rows per
+  // band.  Bands
+  // multiply the blocks of a launch that alone would leave the GPU part idle,
+  // but each costs partial sums and strip set-up.  Forward ms over 4 / 8 / 16
+  // / 32 bands (16-angle blocks, compacted): one 2048^2 slice, 1440 angles:
+  // fanbeam 2.96 / 2.71 / 2.61 / 2.78, parallel beam 2.86 / 2.49 / 2.32 / 2.37;
+  // 32 such slices: fanbeam 71.0 / 72.0 / 74.5 / -, parallel beam 70.1 /
+  // 69.5 / 70.0 / -; one 512^2 fan slice, 360 angles: - / 0.158 / 0.142 / 0.144.
+  const int64_t gb = group_blocks(L.Ns, L.Na, K) * L.Nz;
+  int bands = 4;
+  while (bands < 32 && gb * bands < 12288 && (bands < 8 || std::min(L.Nx, L.Ny) >= 32 * bands))
+    bands *= 2;
   // compaction pays once the prologue's few microseconds are small against
   // the projection
   return {K, bands, 4, w >= 1e9};

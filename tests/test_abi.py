@@ -208,16 +208,18 @@ def test_projection_shape_reports_the_automatic_choice(lib):
         shape = (ctypes.c_int * 4)()
         # the benchmark stacks: 32 x 2048^2, 1440 angles, 2560 detectors
         assert lib.gt_projection_shape(1, 0, 2048, 2048, 2560, 1440, 32, shape) == _hip.GT_OK
-        assert list(shape) == [16, 16, 4, 1]
+        assert list(shape) == [16, 4, 4, 1]
         assert lib.gt_projection_shape(0, 0, 2048, 2048, 2560, 1440, 32, shape) == _hip.GT_OK
+        assert list(shape) == [16, 4, 4, 1]
+        # one such slice: 16 bands (too few blocks with 4); one 512^2 fan slice: 32
+        assert lib.gt_projection_shape(1, 0, 2048, 2048, 2560, 1440, 1, shape) == _hip.GT_OK
         assert list(shape) == [16, 16, 4, 1]
+        assert lib.gt_projection_shape(1, 0, 512, 512, 768, 360, 1, shape) == _hip.GT_OK
+        assert list(shape) == [8, 32, 4, 0]
         assert lib.gt_projection_shape(1, 1, 2048, 2048, 2560, 1440, 32, shape) == _hip.GT_OK
         assert list(shape) == [8, 1, 16, 32]
         assert lib.gt_projection_shape(0, 1, 2048, 2048, 2560, 1440, 32, shape) == _hip.GT_OK
         assert list(shape) == [8, 1, 16, 16]
-        # 16 bands only while their partials stay under 2^31 elements
-        assert lib.gt_projection_shape(1, 0, 2048, 2048, 2560, 1440, 64, shape) == _hip.GT_OK
-        assert list(shape) == [16, 8, 4, 1]
         # tiny calls: 4 bins per lane, per-pixel reads, no compaction; unstaged back
         assert lib.gt_projection_shape(0, 0, 32, 32, 48, 30, 1, shape) == _hip.GT_OK
         assert list(shape) == [4, 8, 1, 0]
