@@ -283,6 +283,27 @@ __device__ __forceinline__ void advance(GPtr<real> &px) {
   asm volatile("" : "+v"(px));
 }
 
+// Strip walks over a buffer descriptor (BUFW): the two row-padded image
+// copies lie in one workspace block starting at imgX, and a lane walks a row
+// by a 32-bit byte offset into it, so the per-chunk address step and loop
+// test are 32-bit operations instead of 64-bit pointer arithmetic.  The
+// launch takes this form only where the block is under 2^31 bytes
+// (strip_buffer_fits); offsets past its end would read zeros.
+template <typename real>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t strip_buffer(const Job<real> &J) {
+  const int64_t bytes = (int64_t)((const char *)(J.imgY + (int64_t)J.Nz * J.sliceY) -
+                                  (const char *)J.imgX);
+  return __builtin_amdgcn_make_buffer_rsrc((void *)J.imgX, (short)0, (int)bytes, 0x00020000);
+}
+
+inline bool strip_buffer_fits(size_t bytes) { return bytes < ((size_t)1 << 31); }
+
+template <typename real, int CH>
+__device__ __forceinline__ RowChunk<real, CH> load_chunk_buf(__amdgpu_buffer_rsrc_t rs, int off) {
+  static_assert(CH * sizeof(real) == 16, "buffer strip walks read 16 bytes per load");
+  return __builtin_bit_cast(RowChunk<real, CH>, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+}
+
 // ---------------------------------------------------------------------------
 // Fan ray geometry of a strip bounded by the rays through detector positions
 // s_lo and s_hi (fanbeam.cl:100-184 with s-1, s+1 replaced by s_lo, s_hi).
@@ -545,7 +566,7 @@ inline int64_t group_blocks(int Ns, int Na, int K) {
 // outside the union strip, so their hat weights land in the discarded slots
 // (0, 1, K + 2, K + 3); past the end of the row they are the copy's zero
 // padding (the single-problem launches pad every row by 4 elements).
-template <typename real, int K, bool BANDED = false, int CH = 1>
+template <typename real, int K, bool BANDED = false, int CH = 1, bool BUFW = false>
 __device__ __forceinline__ void fan_fwd_group(const Job<real> &J, int s0, int a, int z, real *acc,
                                               Band<real> B = {}) {
   using A = Arith<real>;
@@ -592,7 +613,15 @@ __device__ __forceinline__ void fan_fwd_group(const Job<real> &J, int s0, int a,
   const double q00 = qdy * ud0 + c0 * (double)F.RE - qdx * (double)F.midx;
   // row y starts at F.base + y * F.row_stride, walked by one add per row
   const real *row = F.base + (int64_t)ylo * F.row_stride;
-  for (int y = ylo; y <= yhi; ++y, row += F.row_stride) {
+  // BUFW: the same walk by byte offsets from J.imgX
+  [[maybe_unused]] __amdgpu_buffer_rsrc_t rs;
+  [[maybe_unused]] int roff = 0, rstep = 0;
+  if constexpr (BUFW) {
+    rs = strip_buffer(J);
+    roff = (int)((row - J.imgX) * (int64_t)sizeof(real));
+    rstep = (int)(F.row_stride * (int64_t)sizeof(real));
+  }
+  for (int y = ylo; y <= yhi; ++y, row += F.row_stride, roff += rstep) {
     const real w = (real)y - yq;
     const int xh = (int)floor(fmax(fmin(xmax, fma(F.dpx, w, qxm)), (real)-1));
     const int xl = (int)ceil(fmin(fmax((real)0, fma(F.dmx, w, qxm)), (real)F.Nxx));
@@ -600,11 +629,7 @@ __device__ __forceinline__ void fan_fwd_group(const Job<real> &J, int s0, int a,
     const int dy = y - ylo;
     real t = fma(F.ddx, (real)xl, fma(F.ddy, (real)dy, t00));
     real q = (real)fma(qdx, (double)xl, fma(qdy, (double)dy, q00));
-    // walked by address (xl <= xh here): the end pointer is the loop bound
-    const GPtr<real> end = (GPtr<real>)(row + xh);
-    GPtr<real> px = (GPtr<real>)(row + xl);
-    do {
-      const RowChunk<real, CH> c = load_chunk<real, CH>(px);
+    auto chunk = [&](const RowChunk<real, CH> &c) {
 #pragma unroll
       for (int i = 0; i < CH; ++i) {
         const real r = A::rcp(t);
@@ -613,8 +638,23 @@ __device__ __forceinline__ void fan_fwd_group(const Job<real> &J, int s0, int a,
         t += F.ddx;
         q += dq;
       }
-      advance<real, CH>(px);
-    } while (px <= end);
+    };
+    if constexpr (BUFW) {
+      const int end = roff + xh * (int)sizeof(real);
+      int off = roff + xl * (int)sizeof(real);
+      do {
+        chunk(load_chunk_buf<real, CH>(rs, off));
+        off += 16;
+      } while (off <= end);
+    } else {
+      // walked by address (xl <= xh here): the end pointer is the loop bound
+      const GPtr<real> end = (GPtr<real>)(row + xh);
+      GPtr<real> px = (GPtr<real>)(row + xl);
+      do {
+        chunk(load_chunk<real, CH>(px));
+        advance<real, CH>(px);
+      } while (px <= end);
+    }
   }
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -637,7 +677,7 @@ __device__ __forceinline__ real radon_scale(const Job<real> &J) {
 }
 
 // CH: pixels per load along the strip row, as in fan_fwd_group.
-template <typename real, int K, bool BANDED = false, int CH = 1>
+template <typename real, int K, bool BANDED = false, int CH = 1, bool BUFW = false>
 __device__ __forceinline__ void radon_fwd_group(const Job<real> &J, int s0, int a, int z,
                                                 real *acc, Band<real> B = {}) {
   using A = Arith<real>;
@@ -674,7 +714,15 @@ __device__ __forceinline__ void radon_fwd_group(const Job<real> &J, int s0, int 
   const double p00 = (double)off - (double)s0 + 2.0;  // pixel (0, 0) relative to bin s0 - 2
   // row y starts at base + y * row_stride, walked by one add per row
   const real *row = base + (int64_t)ylo * row_stride;
-  for (int y = ylo; y <= yhi; ++y, row += row_stride) {
+  // BUFW: the same walk by byte offsets from J.imgX
+  [[maybe_unused]] __amdgpu_buffer_rsrc_t rs;
+  [[maybe_unused]] int roff = 0, rstep = 0;
+  if constexpr (BUFW) {
+    rs = strip_buffer(J);
+    roff = (int)((row - J.imgX) * (int64_t)sizeof(real));
+    rstep = (int)(row_stride * (int64_t)sizeof(real));
+  }
+  for (int y = ylo; y <= yhi; ++y, row += row_stride, roff += rstep) {
     const real d = (real)y * Y + off;  // detector position of (0, y)
     int xl = (int)((s_first - d) * inv);
     int xh = (int)((s_last - d) * inv);
@@ -685,17 +733,28 @@ __device__ __forceinline__ void radon_fwd_group(const Job<real> &J, int s0, int 
     // evaluated in double: x*X and y*Y + off are up to Ns in size and nearly
     // cancel; inside the strip it lies in [1, K+2], so truncation is floor
     real pr = (real)fma((double)X, (double)xl, fma((double)Y, (double)y, p00));
-    const GPtr<real> end = (GPtr<real>)(row + xh);
-    GPtr<real> px = (GPtr<real>)(row + xl);
-    do {
-      const RowChunk<real, CH> c = load_chunk<real, CH>(px);
+    auto chunk = [&](const RowChunk<real, CH> &c) {
 #pragma unroll
       for (int i = 0; i < CH; ++i) {
         deposit(acc, A::clamp(pr, lo, hi), c.v[i]);
         pr += X;
       }
-      advance<real, CH>(px);
-    } while (px <= end);
+    };
+    if constexpr (BUFW) {
+      const int end = roff + xh * (int)sizeof(real);
+      int o = roff + xl * (int)sizeof(real);
+      do {
+        chunk(load_chunk_buf<real, CH>(rs, o));
+        o += 16;
+      } while (o <= end);
+    } else {
+      const GPtr<real> end = (GPtr<real>)(row + xh);
+      GPtr<real> px = (GPtr<real>)(row + xl);
+      do {
+        chunk(load_chunk<real, CH>(px));
+        advance<real, CH>(px);
+      } while (px <= end);
+    }
   }
   const real scale = radon_scale(J);
 #pragma unroll

@@ -115,6 +115,8 @@ struct FwdPlan {
 // (fanbeam / parallel-beam forward, ms): 4: 78.1 / 72.0, 8: 75.6 / 70.2,
 // 16: 74.4 / 69.8, 32: 75.0 / 69.3, 64: 80.4 / 71.5.
 constexpr int kPlanAngles = kColumnAngles;
+// chunked compacted forwards walk strips by 32-bit offsets where they fit
+constexpr bool kStripBuffer = true;
 constexpr int kPlanCols = kColumnGroups;
 
 // Capacities of a plan: whole columns of active groups per angle, in sets of
@@ -255,7 +257,7 @@ __global__ __launch_bounds__(256) void fwd_plan_kernel(Job<real> J, int K, int n
 // Banded forward over a plan: block blk of band b is a column block while
 // blk < entries / kPlanAngles (its angles: entries kPlanAngles blk + j, lanes
 // by column_lane), then a block of 256 packed lanes.
-template <typename real, bool FAN, int K, int CH>
+template <typename real, bool FAN, int K, int CH, bool BUFW>
 __global__ __launch_bounds__(256) void fwd_band_compact_kernel(Job<real> J, real *part, int nbands,
                                                                FwdPlan P) {
   __shared__ real slots[Slots<K>::kCount * 256];
@@ -281,9 +283,9 @@ __global__ __launch_bounds__(256) void fwd_band_compact_kernel(Job<real> J, real
   real *acc = slots + threadIdx.y * 64 + threadIdx.x;
   const Band<real> B{band, nbands, part};
   if (FAN)
-    fan_fwd_group<real, K, true, CH>(J, g * K, a, blockIdx.y, acc, B);
+    fan_fwd_group<real, K, true, CH, BUFW>(J, g * K, a, blockIdx.y, acc, B);
   else
-    radon_fwd_group<real, K, true, CH>(J, g * K, a, blockIdx.y, acc, B);
+    radon_fwd_group<real, K, true, CH, BUFW>(J, g * K, a, blockIdx.y, acc, B);
 }
 
 template <typename real>
@@ -643,9 +645,11 @@ static Job<real> make_job(bool fan, const Layout &L, const real *ofs, const real
 }
 
 // K = 32 comes with chunked reads (16-byte strip loads) only.
+// bufw: the chunked strip walks go by 32-bit offsets into one buffer
+// descriptor over both image copies (strip_buffer_fits).
 template <typename real, bool FAN, int K>
 static int launch_fwd_group(hipStream_t st, const Job<real> &J, int nb, bool chunk, real *part,
-                            const FwdPlan *plan) {
+                            const FwdPlan *plan, bool bufw) {
   constexpr int CH = 16 / sizeof(real);
   const dim3 block(64, 4);
   if (plan) {
@@ -653,12 +657,15 @@ static int launch_fwd_group(hipStream_t st, const Job<real> &J, int nb, bool chu
     GT_REQUIRE(per_band * nb < ((int64_t)1 << 31), "too many detector groups");
     hipLaunchKernelGGL(fwd_plan_kernel<real>, dim3(nb), dim3(256), 0, st, J, K, nb, *plan);
     const dim3 grid((unsigned)(per_band * nb), J.Nz);
-    if (chunk)
-      hipLaunchKernelGGL((fwd_band_compact_kernel<real, FAN, K, CH>), grid, block, 0, st, J, part,
-                         nb, *plan);
+    if (chunk && bufw)
+      hipLaunchKernelGGL((fwd_band_compact_kernel<real, FAN, K, CH, true>), grid, block, 0, st, J,
+                         part, nb, *plan);
+    else if (chunk)
+      hipLaunchKernelGGL((fwd_band_compact_kernel<real, FAN, K, CH, false>), grid, block, 0, st, J,
+                         part, nb, *plan);
     else if constexpr (K <= 16)
-      hipLaunchKernelGGL((fwd_band_compact_kernel<real, FAN, K, 1>), grid, block, 0, st, J, part,
-                         nb, *plan);
+      hipLaunchKernelGGL((fwd_band_compact_kernel<real, FAN, K, 1, false>), grid, block, 0, st, J,
+                         part, nb, *plan);
     return GT_OK;
   }
   const int64_t blocks = group_blocks(J.Ns, J.Na, K);
@@ -728,6 +735,10 @@ static int fwd_impl(hipStream_t st, real *sino, const real *img, const real *ofs
   J.so = order_strides(sino_order, L.Ns, L.Na, L.Nz);
   const int K = sh.K, nb = sh.bands;
   const bool chunk = sh.chunk > 1;
+  // both padded copies (imgX first, then imgY) in one descriptor's reach
+  const bool bufw = sh.chunk > 1 && kStripBuffer &&
+                    strip_buffer_fits((size_t)((const char *)(J.imgY + (int64_t)L.Nz * J.sliceY) -
+                                               (const char *)J.imgX));
   real *part = nullptr;
   FwdPlan plan{}, *pp = nullptr;
   if (nb > 1) {
@@ -752,15 +763,15 @@ static int fwd_impl(hipStream_t st, real *sino, const real *img, const real *ofs
   } else {
     int rc;
     if (K == 32)
-      rc = launch_fwd_group<real, FAN, 32>(st, J, nb, chunk, part, pp);
+      rc = launch_fwd_group<real, FAN, 32>(st, J, nb, chunk, part, pp, bufw);
     else if (K == 16)
-      rc = launch_fwd_group<real, FAN, 16>(st, J, nb, chunk, part, pp);
+      rc = launch_fwd_group<real, FAN, 16>(st, J, nb, chunk, part, pp, bufw);
     else if (K == 8)
-      rc = launch_fwd_group<real, FAN, 8>(st, J, nb, chunk, part, pp);
+      rc = launch_fwd_group<real, FAN, 8>(st, J, nb, chunk, part, pp, bufw);
     else if (K == 4)
-      rc = launch_fwd_group<real, FAN, 4>(st, J, nb, chunk, part, pp);
+      rc = launch_fwd_group<real, FAN, 4>(st, J, nb, chunk, part, pp, bufw);
     else
-      rc = launch_fwd_group<real, FAN, 2>(st, J, nb, chunk, part, pp);
+      rc = launch_fwd_group<real, FAN, 2>(st, J, nb, chunk, part, pp, bufw);
     if (rc) return rc;
   }
   if (nb > 1)
