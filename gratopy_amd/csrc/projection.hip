@@ -64,7 +64,7 @@ __global__ __launch_bounds__(256) void fwd_ray_kernel(Job<real> J) {
 
 // Bin groups: a lane owns bins [K*g, K*g + K) of angle a (group_lane) and
 // reads its strip rows CH pixels per load.
-template <typename real, bool FAN, int K, int CH>
+template <typename real, bool FAN, int K, int CH, bool BUFW = false>
 __global__ __launch_bounds__(256) void fwd_group_kernel(Job<real> J) {
   __shared__ real slots[Slots<K>::kCount * 256];
   int g;
@@ -72,15 +72,15 @@ __global__ __launch_bounds__(256) void fwd_group_kernel(Job<real> J) {
   if (a >= J.Na) return;
   real *acc = slots + threadIdx.y * 64 + threadIdx.x;
   if (FAN)
-    fan_fwd_group<real, K, false, CH>(J, g * K, a, blockIdx.y, acc);
+    fan_fwd_group<real, K, false, CH, BUFW>(J, g * K, a, blockIdx.y, acc);
   else
-    radon_fwd_group<real, K, false, CH>(J, g * K, a, blockIdx.y, acc);
+    radon_fwd_group<real, K, false, CH, BUFW>(J, g * K, a, blockIdx.y, acc);
 }
 
 // Banded form of the bin-group forward: blockIdx.x = band + nbands * block,
 // so consecutive blocks, which the dispatcher hands to the 8 XCDs in turn,
 // walk different row bands and each XCD's L2 serves one band of the image.
-template <typename real, bool FAN, int K, int CH>
+template <typename real, bool FAN, int K, int CH, bool BUFW = false>
 __global__ __launch_bounds__(256) void fwd_band_kernel(Job<real> J, real *part, int nbands) {
   __shared__ real slots[Slots<K>::kCount * 256];
   const int band = blockIdx.x % nbands;
@@ -90,9 +90,9 @@ __global__ __launch_bounds__(256) void fwd_band_kernel(Job<real> J, real *part, 
   real *acc = slots + threadIdx.y * 64 + threadIdx.x;
   const Band<real> B{band, nbands, part};
   if (FAN)
-    fan_fwd_group<real, K, true, CH>(J, g * K, a, blockIdx.y, acc, B);
+    fan_fwd_group<real, K, true, CH, BUFW>(J, g * K, a, blockIdx.y, acc, B);
   else
-    radon_fwd_group<real, K, true, CH>(J, g * K, a, blockIdx.y, acc, B);
+    radon_fwd_group<real, K, true, CH, BUFW>(J, g * K, a, blockIdx.y, acc, B);
 }
 
 // ---- compacted banded forward ----------------------------------------------
@@ -672,7 +672,12 @@ static int launch_fwd_group(hipStream_t st, const Job<real> &J, int nb, bool chu
   GT_REQUIRE(blocks * 256 < ((int64_t)1 << 31) && blocks * nb < ((int64_t)1 << 31),
              "too many detector groups");
   const dim3 grid((unsigned)(blocks * nb), J.Nz);
-  if (chunk) {
+  if (chunk && bufw) {
+    if (nb > 1)
+      hipLaunchKernelGGL((fwd_band_kernel<real, FAN, K, CH, true>), grid, block, 0, st, J, part, nb);
+    else
+      hipLaunchKernelGGL((fwd_group_kernel<real, FAN, K, CH, true>), grid, block, 0, st, J);
+  } else if (chunk) {
     if (nb > 1)
       hipLaunchKernelGGL((fwd_band_kernel<real, FAN, K, CH>), grid, block, 0, st, J, part, nb);
     else
