@@ -182,8 +182,8 @@ __global__ __launch_bounds__(256) void batch_back_kernel(const Job<real> *__rest
 
 // tile (slice, tile x, tile y): LDS-staged backprojection of TX x (256/TX)*RY pixels
 template <typename real, int RY, int CA, int TX>
-__global__ __launch_bounds__(256) void batch_back_staged_kernel(const Job<real> *__restrict__ jobs,
-                                                                const int4 *__restrict__ tiles) {
+__global__ __launch_bounds__(256, kStagedWavesPerSimd) void batch_back_staged_kernel(
+    const Job<real> *__restrict__ jobs, const int4 *__restrict__ tiles) {
   __shared__ real rows[CA * kStageWidth];
   __shared__ StageWindow<real> win[CA];
   const int4 t = tiles[blockIdx.x];

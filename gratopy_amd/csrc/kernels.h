@@ -1098,6 +1098,14 @@ __device__ __forceinline__ void staged_angle(const AngleRec<real> &c, const Stag
   }
 }
 
+// Occupancy target of the kernels built on back_staged_tile: one block's 16
+// angle windows take ~21 KB of LDS, room for 7 blocks (7 waves per SIMD) per
+// CU.  Left to itself the compiler gives the fanbeam tile 94 VGPRs, which
+// allow 5; held to 7 waves per SIMD (__launch_bounds__(256, 7)) it fits in
+// 72 without spilling, and the 32 x 2048^2 fanbeam backprojection goes from
+// 51.9 to 48.4 ms (parallel beam: under 72 either way).
+constexpr int kStagedWavesPerSimd = 7;
+
 // LDS-staged backprojection of one tile: the block (256 threads) owns the TX x
 // (256/TX)*RY pixels starting at (bx*TX, by*(256/TX)*RY) of slice z; per chunk
 // of CA angles it computes each angle's window (stage_window), copies it into

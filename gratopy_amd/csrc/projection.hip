@@ -343,7 +343,7 @@ __global__ __launch_bounds__(256) void back_reduce_kernel(Job<real> J, const rea
 
 // LDS-staged form (back_staged_tile): one tile of TX x (256/TX)*RY pixels per block.
 template <typename real, bool FAN, int RY, int CA, int TX>
-__global__ __launch_bounds__(256) void back_staged_kernel(Job<real> J) {
+__global__ __launch_bounds__(256, kStagedWavesPerSimd) void back_staged_kernel(Job<real> J) {
   __shared__ real rows[CA * kStageWidth];
   __shared__ StageWindow<real> win[CA];
   back_staged_tile<real, FAN, RY, CA, TX>(J, blockIdx.x, blockIdx.y, blockIdx.z, rows, win);
@@ -353,7 +353,8 @@ __global__ __launch_bounds__(256) void back_staged_kernel(Job<real> J) {
 // back_band_kernel); each block stages and sums the angles of its band into
 // the band partials, which back_reduce_kernel adds in band order.
 template <typename real, bool FAN, int RY, int CA, int TX>
-__global__ __launch_bounds__(256) void back_staged_band_kernel(Job<real> J, real *part, int nbands) {
+__global__ __launch_bounds__(256, kStagedWavesPerSimd) void back_staged_band_kernel(Job<real> J, real *part,
+                                                                                  int nbands) {
   __shared__ real rows[CA * kStageWidth];
   __shared__ StageWindow<real> win[CA];
   const int band = blockIdx.x % nbands;
