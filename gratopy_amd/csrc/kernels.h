@@ -979,7 +979,9 @@ __device__ __forceinline__ void band_back_reduce_entry(const Job<real> &J,
 // taps of every pixel from there, so the per-pixel gathers leave the vector
 // memory pipeline (whose address unit limits the unstaged kernel).
 // ---------------------------------------------------------------------------
-constexpr int kStageWidth = 320;  // LDS elements per staged window
+// LDS elements per staged window: 16 windows of 304 floats and their
+// records take 20032 bytes, so 8 blocks fit a CU's 160 KB
+constexpr int kStageWidth = 304;
 
 // Window of one angle for one tile: padded-sinogram indices [w0, w0 + n),
 // zero outside [0, Ns + 4).  Coordinates are taken relative to the tile's
@@ -1099,12 +1101,14 @@ __device__ __forceinline__ void staged_angle(const AngleRec<real> &c, const Stag
 }
 
 // Occupancy target of the kernels built on back_staged_tile: one block's 16
-// angle windows take ~21 KB of LDS, room for 7 blocks (7 waves per SIMD) per
-// CU.  Left to itself the compiler gives the fanbeam tile 94 VGPRs, which
-// allow 5; held to 7 waves per SIMD (__launch_bounds__(256, 7)) it fits in
-// 72 without spilling, and the 32 x 2048^2 fanbeam backprojection goes from
-// 51.9 to 48.4 ms (parallel beam: under 72 either way).
-constexpr int kStagedWavesPerSimd = 7;
+// angle windows (kStageWidth) take 20 KB of LDS, room for 8 blocks (8 waves
+// per SIMD) per CU.  Left to itself the compiler gives the fanbeam tile 94
+// VGPRs, which allow 5; held to 8 waves per SIMD (__launch_bounds__(256, 8))
+// it fits in 64, with two registers spilled outside the angle loop.  On the
+// 32 x 2048^2 stacks the backprojection goes from 51.9 to 48.4 ms (fanbeam,
+// 7 waves with 320-element windows) and 48.6 (8 waves); parallel beam 32.7
+// -> 31.9 ms.
+constexpr int kStagedWavesPerSimd = 8;
 
 // LDS-staged backprojection of one tile: the block (256 threads) owns the TX x
 // (256/TX)*RY pixels starting at (bx*TX, by*(256/TX)*RY) of slice z; per chunk
