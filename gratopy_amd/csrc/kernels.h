@@ -1060,43 +1060,63 @@ __device__ __forceinline__ StageWindow<real> stage_window(const Job<real> &J, in
 // Adds one angle's contribution to the RY pixels (xa + dx, ya + dy + r) of
 // this lane.  STAGED: `row` is the angle's LDS window (no clamp); otherwise
 // the padded sinogram row in memory (clamped as in the unstaged kernel).
+// Row r of the lane is one FMA from row 0, as in back_pixels (row 0 is t0, q0
+// themselves: fma(c, 0, t0) is not folded, IEEE zeros and NaNs).
 template <typename real, bool FAN, int RY, bool STAGED>
 __device__ __forceinline__ void staged_angle(const AngleRec<real> &c, const StageWindow<real> &W,
                                              const real *row, real dx, real dy, real *acc) {
   using A = Arith<real>;
   const real t0 = FAN ? fma(c.c[0], dx, fma(c.c[1], dy, W.k0)) : (real)1;
   const real q0 = FAN ? fma(W.k3, dx, fma(W.k4, dy, W.k1)) : fma(c.c[0], dx, fma(c.c[1], dy, W.k0));
+  if constexpr (STAGED) {
+    // In three passes: every row's tap index, then all 2 RY LDS taps, then
+    // the interpolation.  The taps of all rows are then in flight together
+    // instead of one row's at a time.  The LDS-qualified pointer keeps the
+    // compiler from merging these taps with the memory path's global reads
+    // into generic (flat) loads, which would wait on global memory.
+    real wr[RY], fr[RY], v0[RY], v1[RY];
+    int ir[RY];
 #pragma unroll
-  for (int r = 0; r < RY; ++r) {
-    // row r of the lane: one FMA from row 0, as in back_pixels (row 0 is
-    // t0, q0 themselves: fma(c, 0, t0) is not folded, IEEE zeros and NaNs)
-    const real t = FAN ? (r == 0 ? t0 : fma(c.c[1], (real)r, t0)) : (real)1;
-    const real q = r == 0 ? q0 : fma(FAN ? W.k4 : c.c[1], (real)r, q0);
-    real p, w = 1;
-    if (FAN) {
-      w = A::rcp(t);
-      p = STAGED ? q * w : fma(q, w, W.k2);
-    } else {
-      p = q;
+    for (int r = 0; r < RY; ++r) {
+      const real t = FAN ? (r == 0 ? t0 : fma(c.c[1], (real)r, t0)) : (real)1;
+      const real q = r == 0 ? q0 : fma(FAN ? W.k4 : c.c[1], (real)r, q0);
+      real p = q;
+      wr[r] = 1;
+      if (FAN) {
+        wr[r] = A::rcp(t);
+        p = q * wr[r];
+      }
+      fr[r] = A::fract(p);
+      ir[r] = (int)p;
     }
-    if (!STAGED) p = A::clamp(p, W.lo, W.hi);
-    const real f = A::fract(p);
-    real v0, v1;
-    if (STAGED) {
-      // An LDS-qualified pointer: the compiler cannot merge these taps with the
-      // global reads of the direct path into one generic (flat) load, which
-      // would wait on global memory for every LDS tap.
-      const __attribute__((address_space(3))) real *tap =
-          (const __attribute__((address_space(3))) real *)row + (int)p;
-      v0 = tap[0];
-      v1 = tap[1];
-    } else {
+    const __attribute__((address_space(3))) real *win =
+        (const __attribute__((address_space(3))) real *)row;
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      v0[r] = win[ir[r]];
+      v1[r] = win[ir[r] + 1];
+    }
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      const real v = v0[r] + fr[r] * (v1[r] - v0[r]);
+      acc[r] = FAN ? acc[r] + v * wr[r] : acc[r] + v;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      const real t = FAN ? (r == 0 ? t0 : fma(c.c[1], (real)r, t0)) : (real)1;
+      const real q = r == 0 ? q0 : fma(FAN ? W.k4 : c.c[1], (real)r, q0);
+      real p = q, w = 1;
+      if (FAN) {
+        w = A::rcp(t);
+        p = fma(q, w, W.k2);
+      }
+      p = A::clamp(p, W.lo, W.hi);
+      const real f = A::fract(p);
       const real *tap = row + (int)p;
-      v0 = tap[0];
-      v1 = tap[1];
+      const real v = tap[0] + f * (tap[1] - tap[0]);
+      acc[r] = FAN ? acc[r] + v * w : acc[r] + v;
     }
-    const real v = v0 + f * (v1 - v0);
-    acc[r] = FAN ? acc[r] + v * w : acc[r] + v;
   }
 }
 
@@ -1104,7 +1124,7 @@ __device__ __forceinline__ void staged_angle(const AngleRec<real> &c, const Stag
 // angle windows (kStageWidth) take 20 KB of LDS, room for 8 blocks (8 waves
 // per SIMD) per CU.  Left to itself the compiler gives the fanbeam tile 94
 // VGPRs, which allow 5; held to 8 waves per SIMD (__launch_bounds__(256, 8))
-// it fits in 64, with two registers spilled outside the angle loop.  On the
+// it fits in 64, with a few registers spilled outside the angle loop.  On the
 // 32 x 2048^2 stacks the backprojection goes from 51.9 to 48.4 ms (fanbeam,
 // 7 waves with 320-element windows) and 48.6 (8 waves); parallel beam 32.7
 // -> 31.9 ms.
@@ -1144,6 +1164,10 @@ __device__ __forceinline__ void back_staged_tile(const Job<real> &J, int bx, int
     const int na = min(CA, a_end - a0);
     if (tid < na) win[tid] = stage_window<real, FAN>(J, a0 + tid, xa, xb, ya, yb);
     __syncthreads();
+    // every angle of the chunk staged (a wave-uniform vote): the accumulation
+    // loop below then runs without the per-angle branch to the memory path
+    const int lane = tid & 63;
+    const bool all_staged = __ballot(lane < na && win[lane < na ? lane : 0].n == 0) == 0;
     for (int k = tid >> 6; k < na; k += 4) {
       const int n = win[k].n, w0 = win[k].w0;
       const real *src = Sz + (int64_t)(a0 + k) * Nsp;
@@ -1153,7 +1177,11 @@ __device__ __forceinline__ void back_staged_tile(const Job<real> &J, int bx, int
       }
     }
     __syncthreads();
-    if (active) {
+    if (active && all_staged) {
+      for (int k = 0; k < na; ++k)
+        staged_angle<real, FAN, RY, true>(J.tab[a0 + k], win[k], rows + k * kStageWidth, dx, dy,
+                                          acc);
+    } else if (active) {
       for (int k = 0; k < na; ++k) {
         const StageWindow<real> W = win[k];
         const AngleRec<real> &c = J.tab[a0 + k];
