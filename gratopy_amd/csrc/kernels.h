@@ -641,11 +641,24 @@ __device__ __forceinline__ void fan_fwd_group(const Job<real> &J, int s0, int a,
     };
     if constexpr (BUFW) {
       const int end = roff + xh * (int)sizeof(real);
+      // two chunks per turn, each loaded one chunk ahead of its use; the
+      // last load may read past the row (the next row, or zeros past the
+      // buffer end) and is not used.  The scheduling barriers keep the
+      // compiler from sinking each load below the work it should overlap.
       int off = roff + xl * (int)sizeof(real);
-      do {
-        chunk(load_chunk_buf<real, CH>(rs, off));
+      RowChunk<real, CH> c0 = load_chunk_buf<real, CH>(rs, off);
+      for (;;) {
+        const RowChunk<real, CH> c1 = load_chunk_buf<real, CH>(rs, off + 16);
+        __builtin_amdgcn_sched_barrier(0);
+        chunk(c0);
         off += 16;
-      } while (off <= end);
+        if (off > end) break;
+        c0 = load_chunk_buf<real, CH>(rs, off + 16);
+        __builtin_amdgcn_sched_barrier(0);
+        chunk(c1);
+        off += 16;
+        if (off > end) break;
+      }
     } else {
       // walked by address (xl <= xh here): the end pointer is the loop bound
       const GPtr<real> end = (GPtr<real>)(row + xh);
@@ -742,11 +755,24 @@ __device__ __forceinline__ void radon_fwd_group(const Job<real> &J, int s0, int 
     };
     if constexpr (BUFW) {
       const int end = roff + xh * (int)sizeof(real);
+      // two chunks per turn, each loaded one chunk ahead of its use; the
+      // last load may read past the row (the next row, or zeros past the
+      // buffer end) and is not used.  The scheduling barriers keep the
+      // compiler from sinking each load below the work it should overlap.
       int o = roff + xl * (int)sizeof(real);
-      do {
-        chunk(load_chunk_buf<real, CH>(rs, o));
+      RowChunk<real, CH> c0 = load_chunk_buf<real, CH>(rs, o);
+      for (;;) {
+        const RowChunk<real, CH> c1 = load_chunk_buf<real, CH>(rs, o + 16);
+        __builtin_amdgcn_sched_barrier(0);
+        chunk(c0);
         o += 16;
-      } while (o <= end);
+        if (o > end) break;
+        c0 = load_chunk_buf<real, CH>(rs, o + 16);
+        __builtin_amdgcn_sched_barrier(0);
+        chunk(c1);
+        o += 16;
+        if (o > end) break;
+      }
     } else {
       const GPtr<real> end = (GPtr<real>)(row + xh);
       GPtr<real> px = (GPtr<real>)(row + xl);

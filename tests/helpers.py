@@ -82,14 +82,10 @@ def rel_err(got, ref):
 
 
 def record_parity(config, quantity, **values):
-    """Print one measured set of parity numbers and append it as a JSON line to
-    gpurun_out/parity_records.jsonl (collected into profiles/ by hand)."""
+    """Print one measured set of parity numbers as a line 'parity {json}'
+    (run pytest with -s; profiles/parity_configs.txt collects these lines
+    from a run's output)."""
     import json
-    import os
 
     line = json.dumps({"config": config, "quantity": quantity, **values}, default=float)
     print("parity", line, flush=True)
-    out = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
-    os.makedirs(out, exist_ok=True)
-    with open(os.path.join(out, "parity_records.jsonl"), "a") as fh:
-        fh.write(line + "\n")
