@@ -915,6 +915,15 @@ __device__ __forceinline__ Taps<real> buffer_taps(__amdgpu_buffer_rsrc_t rs, int
   return t;
 }
 
+// One element at byte offset voff of a raw buffer (zero outside it).
+template <typename real>
+__device__ __forceinline__ real buffer_elem(__amdgpu_buffer_rsrc_t rs, int voff) {
+  if constexpr (sizeof(real) == 4)
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff, 0, 0));
+  else
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, 0, 0));
+}
+
 inline bool buffer_fits(size_t elem, int Ns, int Na) {
   return (size_t)Na * (size_t)(Ns + 4) * elem < ((size_t)1 << 31);
 }
@@ -1059,7 +1068,9 @@ __device__ __forceinline__ StageWindow<real> stage_window(const Job<real> &J, in
   }
   StageWindow<real> W;
   const double span = pmax - pmin;
-  ok = ok && span == span && span < kStageWidth - 4 && fabs(pmin) < 1e9;
+  // |pmin| < 1e8 keeps the staging copy's byte offsets sizeof(real) * (w0 + j)
+  // within int
+  ok = ok && span == span && span < kStageWidth - 4 && fabs(pmin) < 1e8;
   const int w0 = ok ? (int)floor(pmin) - 1 : 0;
   W.w0 = w0;
   W.n = ok ? (int)floor(pmax) + 3 - w0 : 0;
@@ -1194,13 +1205,17 @@ __device__ __forceinline__ void back_staged_tile(const Job<real> &J, int bx, int
     // loop below then runs without the per-angle branch to the memory path
     const int lane = tid & 63;
     const bool all_staged = __ballot(lane < na && win[lane < na ? lane : 0].n == 0) == 0;
-    for (int k = tid >> 6; k < na; k += 4) {
+    // wave k copies windows k, k + 4, ...: a buffer descriptor over the one
+    // padded row makes every index outside [0, Nsp) read zero (a negative
+    // offset is a large unsigned one), with no test per element.  The wave
+    // index goes through readfirstlane so the compiler knows it is uniform
+    // and keeps the descriptor in scalar registers.
+    for (int k = __builtin_amdgcn_readfirstlane(tid >> 6); k < na; k += 4) {
       const int n = win[k].n, w0 = win[k].w0;
-      const real *src = Sz + (int64_t)(a0 + k) * Nsp;
-      for (int j = tid & 63; j < n; j += 64) {
-        const int g = w0 + j;
-        rows[k * kStageWidth + j] = (g >= 0 && g < Nsp) ? src[g] : (real)0;
-      }
+      const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+          (void *)(Sz + (int64_t)(a0 + k) * Nsp), (short)0, Nsp * (int)sizeof(real), 0x00020000);
+      for (int j = tid & 63; j < n; j += 64)
+        rows[k * kStageWidth + j] = buffer_elem<real>(rs, (w0 + j) * (int)sizeof(real));
     }
     __syncthreads();
     if (active && all_staged) {
