@@ -1209,13 +1209,22 @@ __device__ __forceinline__ void back_staged_tile(const Job<real> &J, int bx, int
     // padded row makes every index outside [0, Nsp) read zero (a negative
     // offset is a large unsigned one), with no test per element.  The wave
     // index goes through readfirstlane so the compiler knows it is uniform
-    // and keeps the descriptor in scalar registers.
+    // and keeps the descriptor in scalar registers.  The range check takes
+    // the register offset plus the instruction's offset field without
+    // 32-bit wrap-around, so the empty asm keeps the compiler from moving a
+    // constant part of the offset into that field: -4 + 256 would then read
+    // as out of range instead of as element 63.  (Not volatile: a volatile
+    // asm counts as a possible memory write, and the angle table's loads
+    // after it would no longer be scalar.)
     for (int k = __builtin_amdgcn_readfirstlane(tid >> 6); k < na; k += 4) {
       const int n = win[k].n, w0 = win[k].w0;
       const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
           (void *)(Sz + (int64_t)(a0 + k) * Nsp), (short)0, Nsp * (int)sizeof(real), 0x00020000);
-      for (int j = tid & 63; j < n; j += 64)
-        rows[k * kStageWidth + j] = buffer_elem<real>(rs, (w0 + j) * (int)sizeof(real));
+      for (int j = tid & 63; j < n; j += 64) {
+        int off = (w0 + j) * (int)sizeof(real);
+        asm("" : "+v"(off));
+        rows[k * kStageWidth + j] = buffer_elem<real>(rs, off);
+      }
     }
     __syncthreads();
     if (active && all_staged) {
