@@ -134,6 +134,39 @@ def test_backward_variants(variant, restore_variant):
         _check_wide(ps, sino, True, (geometry, nd, variant))
 
 
+STAGED_VARIANTS = [v for v in autotune.BACK_VARIANTS if v["back_stage"]]
+
+# Detectors narrower than the image, shifted, or both: most tiles' windows
+# start before the padded sinogram row (negative indices) or end past it.
+EDGE_CASES = [
+    ("radon", (64, 48), 90, 16, dict(detector_width=0.5)),
+    ("radon", (64, 48), 45, 24, dict(detector_width=0.8, detector_shift=0.6)),
+    ("fan", (64, 48), 60, 20, dict(R=300.0, RE=150.0, detector_width=40.0, detector_shift=15.0)),
+]
+
+
+@pytest.mark.parametrize("variant", STAGED_VARIANTS,
+                         ids=[autotune.variant_key(v) for v in STAGED_VARIANTS])
+def test_staged_windows_past_the_row_ends(variant, restore_variant):
+    """The window copies of the staged backprojection read the padded row
+    through a buffer descriptor whose range check supplies the zeros outside
+    it, negative indices included.  A build that let the compiler split such
+    an offset into a negative register part and a positive instruction offset
+    read zeros where the row has data (float32 error 3e-3 against the
+    oracle)."""
+    print(f"variant {autotune.variant_key(variant)} edge windows", flush=True)
+    autotune.set_variant(**variant)
+    for geometry, shape, na, nd, kw in EDGE_CASES:
+        g = gratopy.RADON if geometry == "radon" else gratopy.FANBEAM
+        ps = gratopy.ProjectionSettings("cuda:0", g, shape, na, nd, **kw)
+        sino = np.random.default_rng(7).random(ps.sinogram_shape + (2,))
+        for dtype in (F32, F64):
+            ref = _oracle(ps, sino, dtype, True)
+            got = _hip(ps, sino, dtype, True, in_order="C", out_order="C")
+            scale = max(np.abs(ref).max(), 1e-3)
+            assert np.abs(got - ref).max() <= TOL[dtype] * scale, (geometry, nd, variant, dtype)
+
+
 _LARGE = {}
 
 
