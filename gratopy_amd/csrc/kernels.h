@@ -1181,6 +1181,7 @@ __device__ __forceinline__ void back_staged_tile(const Job<real> &J, int bx, int
                                                  real *rows, StageWindow<real> *win, int a_begin = 0,
                                                  int a_end = -1, real *part = nullptr,
                                                  int band = 0) {
+  static_assert(CA % 4 == 0 && CA <= 64, "each of the 4 waves copies CA / 4 windows");
   if (a_end < 0) a_end = J.Na;
   constexpr int TY = 256 / TX;  // lane rows
   const int tid = threadIdx.x;
@@ -1216,14 +1217,39 @@ __device__ __forceinline__ void back_staged_tile(const Job<real> &J, int bx, int
     // as out of range instead of as element 63.  (Not volatile: a volatile
     // asm counts as a possible memory write, and the angle table's loads
     // after it would no longer be scalar.)
-    for (int k = __builtin_amdgcn_readfirstlane(tid >> 6); k < na; k += 4) {
-      const int n = win[k].n, w0 = win[k].w0;
-      const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-          (void *)(Sz + (int64_t)(a0 + k) * Nsp), (short)0, Nsp * (int)sizeof(real), 0x00020000);
-      for (int j = tid & 63; j < n; j += 64) {
-        int off = (w0 + j) * (int)sizeof(real);
-        asm("" : "+v"(off));
-        rows[k * kStageWidth + j] = buffer_elem<real>(rs, off);
+    // Each step loads one element of all the wave's windows before storing
+    // any, so their loads are in flight together; the asm on each loaded
+    // value keeps the compiler from sinking a load into its conditional store.
+    {
+      constexpr int Q = CA / 4;  // windows per wave
+      const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+      __amdgpu_buffer_rsrc_t rs[Q];
+      int n[Q], w0[Q], nmax = 0;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int k = wv + 4 * q;
+        const bool has = k < na;
+        n[q] = has ? win[k].n : 0;
+        w0[q] = has ? win[k].w0 : 0;
+        rs[q] = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(Sz + (int64_t)(a0 + (has ? k : 0)) * Nsp), (short)0, Nsp * (int)sizeof(real),
+            0x00020000);
+        nmax = max(nmax, n[q]);
+      }
+      for (int base = 0; base < nmax; base += 64) {
+        const int j = base + lane;
+        real v[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          int off = (w0[q] + j) * (int)sizeof(real);
+          asm("" : "+v"(off));
+          v[q] = buffer_elem<real>(rs[q], off);
+        }
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          asm("" : "+v"(v[q]));
+          if (j < n[q]) rows[(wv + 4 * q) * kStageWidth + j] = v[q];
+        }
       }
     }
     __syncthreads();
