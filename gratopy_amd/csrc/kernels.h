@@ -1157,9 +1157,11 @@ __device__ __forceinline__ void staged_angle(const AngleRec<real> &c, const Stag
   }
 }
 
-// Occupancy target of the kernels built on back_staged_tile: one block's 16
-// angle windows (kStageWidth) take 20 KB of LDS, room for 8 blocks (8 waves
-// per SIMD) per CU.  Left to itself the compiler gives the fanbeam tile 94
+// Occupancy target of the whole-grid kernels built on back_staged_tile
+// (back_staged_kernel, batch_back_staged_kernel; not the angle-banded form,
+// whose grids are too small to fill it): one block's 16 angle windows
+// (kStageWidth) take 20 KB of LDS, room for 8 blocks (8 waves per SIMD) per
+// CU.  Left to itself the compiler gives the fanbeam tile 94
 // VGPRs, which allow 5; held to 8 waves per SIMD (__launch_bounds__(256, 8))
 // it fits in 64, with a few registers spilled outside the angle loop.  On the
 // 32 x 2048^2 stacks the backprojection goes from 51.9 to 48.4 ms (fanbeam,

@@ -351,10 +351,12 @@ __global__ __launch_bounds__(256, kStagedWavesPerSimd) void back_staged_kernel(J
 
 // LDS-staged and angle-banded: blockIdx.x = band + nbands * tile column (as
 // back_band_kernel); each block stages and sums the angles of its band into
-// the band partials, which back_reduce_kernel adds in band order.
+// the band partials, which back_reduce_kernel adds in band order.  This form
+// serves calls with few tiles, whose grids hold fewer blocks than a CU can
+// take; the occupancy cap of back_staged_kernel would only cost registers
+// here (a 512^2 fan slice: 61 instead of 78 us uncapped).
 template <typename real, bool FAN, int RY, int CA, int TX>
-__global__ __launch_bounds__(256, kStagedWavesPerSimd) void back_staged_band_kernel(Job<real> J, real *part,
-                                                                                  int nbands) {
+__global__ __launch_bounds__(256) void back_staged_band_kernel(Job<real> J, real *part, int nbands) {
   __shared__ real rows[CA * kStageWidth];
   __shared__ StageWindow<real> win[CA];
   const int band = blockIdx.x % nbands;
