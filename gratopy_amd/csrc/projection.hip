@@ -351,12 +351,9 @@ __global__ __launch_bounds__(256, kStagedWavesPerSimd) void back_staged_kernel(J
 
 // LDS-staged and angle-banded: blockIdx.x = band + nbands * tile column (as
 // back_band_kernel); each block stages and sums the angles of its band into
-// the band partials, which back_reduce_kernel adds in band order.  This form
-// serves calls with few tiles, whose grids hold fewer blocks than a CU can
-// take; the occupancy cap of back_staged_kernel would only cost registers
-// here (a 512^2 fan slice: 61 instead of 78 us uncapped).
+// the band partials, which back_reduce_kernel adds in band order.
 template <typename real, bool FAN, int RY, int CA, int TX>
-__global__ __launch_bounds__(256) void back_staged_band_kernel(Job<real> J, real *part, int nbands) {
+__device__ __forceinline__ void back_staged_band_body(const Job<real> &J, real *part, int nbands) {
   __shared__ real rows[CA * kStageWidth];
   __shared__ StageWindow<real> win[CA];
   const int band = blockIdx.x % nbands;
@@ -366,13 +363,36 @@ __global__ __launch_bounds__(256) void back_staged_band_kernel(Job<real> J, real
                                           a0, a1, part, band);
 }
 
+// Grids of at least kStagedCapBlocks blocks: held to kStagedWavesPerSimd, as
+// back_staged_kernel (a 2048^2 fan slice, 16384 blocks: 1.50 against 1.58 ms
+// uncapped).
+template <typename real, bool FAN, int RY, int CA, int TX>
+__global__ __launch_bounds__(256, kStagedWavesPerSimd) void back_staged_band_kernel(Job<real> J, real *part,
+                                                                                    int nbands) {
+  back_staged_band_body<real, FAN, RY, CA, TX>(J, part, nbands);
+}
+
+// Smaller grids leave CUs short of blocks, so the cap would only cost
+// registers: uncapped (a 512^2 fan slice, 1024 blocks: 61 against 78 us).
+template <typename real, bool FAN, int RY, int CA, int TX>
+__global__ __launch_bounds__(256) void back_staged_band_small_kernel(Job<real> J, real *part, int nbands) {
+  back_staged_band_body<real, FAN, RY, CA, TX>(J, part, nbands);
+}
+
+// 8 blocks per CU of the 256
+constexpr int64_t kStagedCapBlocks = 2048;
+
 template <typename real, bool FAN, int RY, int CA, int TX>
 static void launch_staged_tx(hipStream_t st, const Job<real> &J, int nbands, real *part) {
   const dim3 grid(((J.Nx + TX - 1) / TX) * nbands, (J.Ny + (256 / TX) * RY - 1) / ((256 / TX) * RY),
                   J.Nz);
   if (nbands > 1) {
-    hipLaunchKernelGGL((back_staged_band_kernel<real, FAN, RY, CA, TX>), grid, dim3(256), 0, st, J,
-                       part, nbands);
+    if ((int64_t)grid.x * grid.y * grid.z >= kStagedCapBlocks)
+      hipLaunchKernelGGL((back_staged_band_kernel<real, FAN, RY, CA, TX>), grid, dim3(256), 0, st, J,
+                         part, nbands);
+    else
+      hipLaunchKernelGGL((back_staged_band_small_kernel<real, FAN, RY, CA, TX>), grid, dim3(256), 0,
+                         st, J, part, nbands);
     hipLaunchKernelGGL(back_reduce_kernel<real>, dim3((J.Nx + 15) / 16, (J.Ny + 15) / 16, J.Nz),
                        dim3(256), 0, st, J, part, nbands);
   } else {
