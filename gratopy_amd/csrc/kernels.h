@@ -1208,20 +1208,20 @@ __device__ __forceinline__ void back_staged_tile(const Job<real> &J, int bx, int
     // loop below then runs without the per-angle branch to the memory path
     const int lane = tid & 63;
     const bool all_staged = __ballot(lane < na && win[lane < na ? lane : 0].n == 0) == 0;
-    // wave k copies windows k, k + 4, ...: a buffer descriptor over the one
+    // Wave w copies windows w, w + 4, ...  A buffer descriptor over each
     // padded row makes every index outside [0, Nsp) read zero (a negative
-    // offset is a large unsigned one), with no test per element.  The wave
-    // index goes through readfirstlane so the compiler knows it is uniform
-    // and keeps the descriptor in scalar registers.  The range check takes
-    // the register offset plus the instruction's offset field without
-    // 32-bit wrap-around, so the empty asm keeps the compiler from moving a
-    // constant part of the offset into that field: -4 + 256 would then read
-    // as out of range instead of as element 63.  (Not volatile: a volatile
-    // asm counts as a possible memory write, and the angle table's loads
-    // after it would no longer be scalar.)
-    // Each step loads one element of all the wave's windows before storing
-    // any, so their loads are in flight together; the asm on each loaded
-    // value keeps the compiler from sinking a load into its conditional store.
+    // offset is a large unsigned one), with no test per element.  Each step
+    // loads one element of each of the wave's windows before storing any, so
+    // those loads are in flight together.  The wave index goes through
+    // readfirstlane so the compiler knows it is uniform and keeps the
+    // descriptors in scalar registers.  The range check takes the register
+    // offset plus the instruction's offset field without 32-bit wrap-around:
+    // the empty asm on each offset keeps the compiler from moving a constant
+    // part of it into that field (-4 + 256 would then read as out of range
+    // instead of as element 63).  The empty asm on each loaded value keeps a
+    // load from being sunk into its conditional store.  Neither asm is
+    // volatile: a volatile asm counts as a possible memory write, and the
+    // angle table's loads after it would no longer be scalar.
     {
       constexpr int Q = CA / 4;  // windows per wave
       const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
