@@ -115,8 +115,6 @@ struct FwdPlan {
 // (fanbeam / parallel-beam forward, ms): 4: 78.1 / 72.0, 8: 75.6 / 70.2,
 // 16: 74.4 / 69.8, 32: 75.0 / 69.3, 64: 80.4 / 71.5.
 constexpr int kPlanAngles = kColumnAngles;
-// chunked compacted forwards walk strips by 32-bit offsets where they fit
-constexpr bool kStripBuffer = true;
 constexpr int kPlanCols = kColumnGroups;
 
 // Capacities of a plan: whole columns of active groups per angle, in sets of
@@ -764,7 +762,7 @@ static int fwd_impl(hipStream_t st, real *sino, const real *img, const real *ofs
   const int K = sh.K, nb = sh.bands;
   const bool chunk = sh.chunk > 1;
   // both padded copies (imgX first, then imgY) in one descriptor's reach
-  const bool bufw = sh.chunk > 1 && kStripBuffer &&
+  const bool bufw = sh.chunk > 1 &&
                     strip_buffer_fits((size_t)((const char *)(J.imgY + (int64_t)L.Nz * J.sliceY) -
                                                (const char *)J.imgX));
   real *part = nullptr;
